@@ -1,0 +1,235 @@
+"""Full-scan benchmark: FP32 forward + volume VJP + 6-DoF pose VJP.
+
+Workload: 256^3 FP32 seeded ellipsoid phantom at 1 mm, circular orbit of 360
+views (angles 2*pi*k/360), SOD/SDD 1000/1500 mm, 512x512 detector with the
+full-support coverage pitch, 1024 clipped midpoint samples per ray.  One step
+is one iteration as the reference benchmark defines it
+(benchmarks/fullscan.py:224-229): a fresh Projector forward at pose tangent
+xi = 0, the squared-error residual against a seeded target image, and
+torch.autograd.grad for the field and the FP64 pose together.
+
+Prints one JSON line.  ``value`` = nominal samples per pass (360*512^2*1024)
+divided by the median iteration time, in Gsamples/s.  ``roofline`` is for the
+dominant kernel, timed with HIP events on its launch stream inside the timed
+region.  ``cpu_baseline`` times the CPU oracle (a restatement of the reference
+FP32 algorithm) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+SHAPE = (256, 256, 256)
+VIEWS = 360
+DETECTOR = (512, 512)
+SAMPLES = 1024
+SOD, SDD = 1000.0, 1500.0
+# algorithmic bytes per effective sample (FP32 storage): 8 corners x 4 B
+GATHER_B = 32
+SCATTER_B = 32
+
+
+def _args():
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=5)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--views", type=int, default=VIEWS)
+    parser.add_argument("--samples", type=int, default=SAMPLES)
+    parser.add_argument("--storage", default="float32", choices=["float32", "bfloat16", "float16"])
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--cpu-rows-stride", type=int, default=8)
+    return parser.parse_args()
+
+
+def _log(message: str) -> None:
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
+    """Time the CPU oracle (FP32 restatement) on views {0,90,180,270}, every
+    ``stride``-th detector row, full columns and samples: forward + joint VJP."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    picks = [v for v in (0, 90, 180, 270) if v < len(geometry.views)]
+    panels = []
+    for k in picks:
+        p = geometry.views[k]
+        rows = (p.shape[0] + stride - 1) // stride
+        panels.append(O.Panel(p.source_mm, p.origin_mm, p.u, p.v, (p.spacing_mm[0], p.spacing_mm[1] * stride), (rows, p.shape[1])))
+    scene = O.Scene(O.Grid(grid.shape, grid.spacing_mm, grid.origin_mm, grid.orientation), samples, panels=panels)
+    t0 = time.perf_counter()
+    image, _ = scene.forward(values)
+    seeds = (image - 0.5 * image.mean()).astype(np.float32)
+    scene.vjp(values, seeds)
+    elapsed = time.perf_counter() - t0
+    rays = len(picks) * panels[0].shape[0] * panels[0].shape[1]
+    nominal = rays * samples
+    return {
+        "value": round(nominal / elapsed / 1e9, 6),
+        "unit": "Gsamples/s",
+        "cores": O.threads(),
+        "kind": "port",
+        "sample": (
+            f"CPU oracle (reference-algorithm FP32 restatement, OpenMP over rays) forward + volume + pose VJP on views "
+            f"{picks}, every {stride}th detector row ({panels[0].shape[0]}x{panels[0].shape[1]} rays/view), "
+            f"{samples} samples/ray: {nominal:.3e} nominal samples in {elapsed:.2f} s"
+        ),
+    }
+
+
+def main() -> int:
+    args = _args()
+    if args.gpus != 1:
+        print(
+            json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                        "n_gpus": args.gpus}),
+            flush=True,
+        )
+        return 2
+    import numpy as np
+    import torch
+
+    from dpt_amd import phantoms
+    from dpt_amd.projection import ProjectionSpec, configuration, prepare
+    from dpt_amd.torch import Projector
+
+    device = torch.device("cuda:0")
+    _log("building phantom and geometry")
+    grid = phantoms.centred_grid(SHAPE)
+    values = phantoms.chest_phantom(grid, seed=2026)
+    geometry, pitch = phantoms.circular_geometry(grid, args.views, DETECTOR, sod=SOD, sdd=SDD)
+    field = torch.tensor(values, device=device, requires_grad=True)
+
+    def make():
+        return Projector(grid=grid, geometry=geometry, precision="fp32", samples_per_ray=args.samples, storage=args.storage)
+
+    # seeded target: the same phantom seen at a seeded pose (+/-2 deg, +/-5 mm) plus noise
+    rng = np.random.default_rng(7)
+    xi_true = np.concatenate([rng.uniform(-5, 5, 3), np.deg2rad(rng.uniform(-2, 2, 3))])
+    with torch.no_grad():
+        target = make()(field.detach(), torch.tensor(xi_true, dtype=torch.float64, device=device))
+        target += 1e-3 * torch.randn(target.shape, generator=torch.Generator(device).manual_seed(11), device=device)
+    torch.cuda.synchronize()
+
+    pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
+    projector = make()
+    needs = 3
+    stream = torch.cuda.current_stream(device).cuda_stream
+    entry_fwd = prepare(projector._configuration, 0, stream, needs)
+    entry_fwd.plan.set_timing(True)
+    poses_packed = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float64, device=device).expand(args.views, 12).contiguous()
+    hits = entry_fwd.plan.count_hit_rays(poses_packed.data_ptr(), stream)
+    nominal = args.views * DETECTOR[0] * DETECTOR[1] * args.samples
+    effective = hits * args.samples
+    _log(f"pitch {pitch:.5f} mm, hit rays {hits} of {args.views * DETECTOR[0] * DETECTOR[1]}")
+
+    def step():
+        module = make()
+        image = module(field, pose)
+        residual = image - target
+        return torch.autograd.grad(image, (field, pose), residual)
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        _log(f"warmup {i + 1}/{args.warmup} done")
+
+    times, fwd_ms, vjp_ms = [], [], []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        grads = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        fwd_ms.append(entry_fwd.plan.kernel_ms(0))
+        vjp_ms.append(entry_fwd.plan.kernel_ms(1))
+        _log(f"step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms (forward kernel {fwd_ms[-1]:.1f} ms, vjp kernel {vjp_ms[-1]:.1f} ms)")
+    total = time.perf_counter() - start
+    assert all(torch.isfinite(g).all() for g in grads)
+
+    median = float(np.median(times))
+    gsamples = nominal / median / 1e9
+    fwd = float(np.mean(fwd_ms))
+    vjp = float(np.mean(vjp_ms))
+    half = args.storage != "float32"
+    gather_b = 16 if half else GATHER_B
+    if vjp >= fwd:
+        kernel, ms, per_sample = "vjp32_kernel (joint volume + pose)", vjp, gather_b + SCATTER_B
+    else:
+        kernel, ms, per_sample = "forward32_kernel", fwd, gather_b
+    achieved = effective * per_sample / (ms * 1e-3) / 1e9
+    traffic = None
+    traffic_file = ROOT / "profiles" / "traffic.json"
+    if traffic_file.exists():
+        try:
+            traffic = json.loads(traffic_file.read_text()).get(kernel.split()[0])
+        except (ValueError, OSError):
+            traffic = None
+    iteration_bytes = effective * (2 * gather_b + SCATTER_B)
+    result = {
+        "metric": "Gsamples/s fwd+vol+pose VJP 256^3/360v/512^2; p50/p99 batched-solve latency",
+        "value": round(gsamples, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3),
+        "median_ms": round(median * 1e3, 3),
+        "p95_ms": round(float(np.percentile(times, 95)) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if not half else f"f32 sampling, {args.storage} storage",
+        "data": "synthetic: seeded ellipsoid phantom (body/2 lungs/5 bones, noise 1e-4), seeded target pose",
+        "config": {
+            "workload": "FP32 forward + volume VJP + 6-DoF pose VJP, squared-error loss",
+            "volume": "256^3 float32 at 1 mm",
+            "views": args.views,
+            "detector": "512x512",
+            "pitch_mm": round(pitch, 6),
+            "sod_sdd_mm": [SOD, SDD],
+            "samples_per_ray": args.samples,
+            "nominal_samples_per_pass": nominal,
+            "effective_samples_per_pass": effective,
+            "storage": args.storage,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_effective_sample": per_sample,
+            "kernel_ms": round(ms, 3),
+            "forward_kernel_ms": round(fwd, 3),
+            "vjp_kernel_ms": round(vjp, 3),
+            "iteration_frac": round(iteration_bytes / median / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "reference_blackwell_iteration_gsamples": 65.5,
+    }
+    if not args.no_cpu_baseline:
+        _log("timing CPU oracle baseline")
+        result["cpu_baseline"] = cpu_baseline(grid, geometry, values, args.samples, args.cpu_rows_stride)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,203 @@
+"""ctypes binding of the C ABI declared in include/dpt_amd.h.
+
+The shared library is built in-tree by ``python -m dpt_amd.build_native``.  The
+product path has no fallback: if the library is missing, every projector call
+raises ``NativeLibraryError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import numpy as np
+
+from .contracts import ContractError
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libdpt_amd.so"
+
+# every symbol include/dpt_amd.h declares
+EXPORTS = (
+    "dpt_plan_create",
+    "dpt_plan_destroy",
+    "dpt_forward",
+    "dpt_vjp",
+    "dpt_status",
+    "dpt_convert_field",
+    "dpt_set_timing",
+    "dpt_kernel_ms",
+    "dpt_count_hit_rays",
+    "dpt_last_error",
+    "dpt_abi_version",
+)
+
+ARITH_FP32, ARITH_FP64 = 0, 1
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
+
+STORAGE_CODES = {"float32": STORE_F32, "bfloat16": STORE_BF16, "float16": STORE_F16}
+ARITH_CODES = {"float32": ARITH_FP32, "float64": ARITH_FP64}
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library is missing or a native call failed."""
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("object_to_grid", ctypes.c_double * 9),
+        ("grid_origin", ctypes.c_double * 3),
+    ]
+
+
+_LIB = None
+_LOCK = threading.Lock()
+
+
+def lib():
+    """Load the HIP library once; raise loudly when it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not LIB_PATH.exists():
+            raise NativeLibraryError(
+                f"dpt_amd HIP library not found at {LIB_PATH}; build it with "
+                "`python -m dpt_amd.build_native` (no CPU fallback exists)"
+            )
+        h = ctypes.CDLL(str(LIB_PATH))
+        vp, i32, u32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64
+        h.dpt_plan_create.argtypes = [
+            ctypes.POINTER(GridDesc), i32, i32, i32, vp, vp, vp, i32, i32, i32, u32, i32,
+            ctypes.POINTER(vp),
+        ]
+        h.dpt_plan_create.restype = ctypes.c_int
+        h.dpt_plan_destroy.argtypes = [vp]
+        h.dpt_plan_destroy.restype = ctypes.c_int
+        h.dpt_forward.argtypes = [vp, vp, vp, vp, vp]
+        h.dpt_forward.restype = ctypes.c_int
+        h.dpt_vjp.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp]
+        h.dpt_vjp.restype = ctypes.c_int
+        h.dpt_status.argtypes = [vp, vp, i32, vp]
+        h.dpt_status.restype = ctypes.c_int
+        h.dpt_convert_field.argtypes = [vp, vp, i64, i32, vp, vp]
+        h.dpt_convert_field.restype = ctypes.c_int
+        h.dpt_set_timing.argtypes = [vp, i32]
+        h.dpt_set_timing.restype = ctypes.c_int
+        h.dpt_kernel_ms.argtypes = [vp, i32]
+        h.dpt_kernel_ms.restype = ctypes.c_double
+        h.dpt_count_hit_rays.argtypes = [vp, vp, vp]
+        h.dpt_count_hit_rays.restype = i64
+        h.dpt_last_error.argtypes = []
+        h.dpt_last_error.restype = ctypes.c_char_p
+        h.dpt_abi_version.argtypes = []
+        h.dpt_abi_version.restype = i32
+        _LIB = h
+        return h
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        message = lib().dpt_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ContractError(f"{what}: {message}")
+        raise NativeLibraryError(f"{what} failed ({rc}): {message}")
+
+
+class Plan:
+    """Owns one native dpt_plan (device metadata, status words, scratch)."""
+
+    def __init__(
+        self,
+        grid,
+        *,
+        views: int,
+        rows: int,
+        cols: int,
+        panels: np.ndarray | None,
+        rays: tuple[np.ndarray, np.ndarray] | None,
+        samples: int,
+        arithmetic: str,
+        storage: str,
+        needs: int,
+        device: int,
+    ) -> None:
+        h = lib()
+        desc = GridDesc()
+        desc.nx, desc.ny, desc.nz = grid.xyz
+        for i, value in enumerate(grid.object_to_grid_matrix()):
+            desc.object_to_grid[i] = value
+        for i in range(3):
+            desc.grid_origin[i] = grid.origin_mm[i]
+        self._keep = []
+        panel_ptr = src_ptr = end_ptr = None
+        if panels is not None:
+            arr = np.ascontiguousarray(panels, dtype=np.float64)
+            self._keep.append(arr)
+            panel_ptr = arr.ctypes.data
+        else:
+            src = np.ascontiguousarray(rays[0], dtype=np.float64)
+            end = np.ascontiguousarray(rays[1], dtype=np.float64)
+            self._keep += [src, end]
+            src_ptr, end_ptr = src.ctypes.data, end.ctypes.data
+        handle = ctypes.c_void_p()
+        rc = h.dpt_plan_create(
+            ctypes.byref(desc), views, rows, cols, panel_ptr, src_ptr, end_ptr, samples,
+            ARITH_CODES[arithmetic], STORAGE_CODES[storage], needs, device, ctypes.byref(handle),
+        )
+        check(rc, "dpt_plan_create")
+        self.handle = handle
+        self.views, self.rows, self.cols = views, rows, cols
+        self.arithmetic, self.storage = arithmetic, storage
+        self.device = device
+        self._codes = np.zeros(views, dtype=np.int32)
+
+    def forward(self, field: int, poses: int, out: int, stream: int) -> None:
+        check(lib().dpt_forward(self.handle, field, poses, out, stream), "dpt_forward")
+
+    def vjp(self, field: int, poses: int, cotangent: int, grad_field: int | None,
+            grad_pose: int | None, flags: int, stream: int) -> None:
+        check(
+            lib().dpt_vjp(self.handle, field, poses, cotangent, grad_field, grad_pose, flags, stream),
+            "dpt_vjp",
+        )
+
+    def status(self, stream: int, clear: bool = True) -> np.ndarray:
+        check(lib().dpt_status(self.handle, self._codes.ctypes.data, 1 if clear else 0, stream), "dpt_status")
+        return self._codes.copy()
+
+    def set_timing(self, enabled: bool) -> None:
+        check(lib().dpt_set_timing(self.handle, 1 if enabled else 0), "dpt_set_timing")
+
+    def kernel_ms(self, kind: int) -> float:
+        return float(lib().dpt_kernel_ms(self.handle, kind))
+
+    def count_hit_rays(self, poses: int, stream: int) -> int:
+        value = int(lib().dpt_count_hit_rays(self.handle, poses, stream))
+        if value < 0:
+            check(value, "dpt_count_hit_rays")
+        return value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().dpt_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def convert_field(src: int, dst: int, count: int, storage: str, status: int | None, stream: int) -> None:
+    check(
+        lib().dpt_convert_field(src, dst, count, STORAGE_CODES[storage], status, stream),
+        "dpt_convert_field",
+    )

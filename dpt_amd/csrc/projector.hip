@@ -1,0 +1,1446 @@
+// MI355X (gfx950) cone-beam projector: forward ray march, volume VJP and
+// 6-DoF pose VJP, FP32 and FP64 sampling, FP32/BF16/FP16 field storage.
+//
+// Semantics follow the reference kernels operation for operation:
+//   FP32 sampling  kernels/projection_fp32.py:59-617
+//   FP64 sampling  kernels/projection.py:79-296, 396-430, 802-1237
+//   ray setup      kernels/projection.py:79-121 (FP64, once per ray)
+//   pose algebra   kernels/geometry.py:13-74
+// The translation unit is compiled with -ffp-contract=off; every FMA below is
+// written explicitly (fmaf/fma).  Where the reference leaves a product-sum
+// unmarked and its code generator (Warp, fuse_fp=True) fuses it, the fused form
+// is written out, so the per-ray results are a fixed function of the inputs.
+//
+// Execution model (not the reference's): one launch covers all views.  A
+// workgroup of 256 lanes owns one 16x16 detector tile of one view (four wave64
+// quadrants of 8x8 rays); workgroup ids are remapped so each XCD walks a
+// contiguous range of tiles.  Pose cotangents are reduced per ray in FP64,
+// across the wave with shuffles and across the tile in LDS, then across tiles
+// in a fixed-order FP64 pass.  Volume cotangents are summed per cell in
+// registers (FP32 increments, reference flush-to-zero rule) and added to an
+// FP64 accumulator with global atomics; a final pass rounds to FP32.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/dpt_amd.h"
+
+namespace dpt {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 16;
+constexpr int kXcds = 8;
+constexpr int kMaxComponents = 12;
+
+struct Params {
+    int nx, ny, nz;
+    int sx, sy, sz;          // corner strides: 1, nx, nx*ny, or 0 for singleton axes
+    float hx, hy, hz;        // n - 0.5 (upper support bound)
+    float cx, cy, cz;        // n - 1 (upper clamp)
+    int capx, capy, capz;    // max(n - 2, 0)
+    double g[9];             // object_to_grid
+    double go[3];            // grid origin
+    int views, rows, cols, samples;
+    int bundle;              // 0: panels, 1: explicit rays
+    int tile_w, tile_h;      // 16x16 for panels, 256x1 for bundles
+    int tiles_x, tiles_y, tiles_per_view, total_tiles, chunk;
+    const double *panels;    // views x 12
+    const double *ray_src;   // views x pixels x 3
+    const double *ray_end;
+    const double *poses;     // views x 12
+    int *status;             // views
+};
+
+// ------------------------------------------------------------------ utils --
+
+__device__ __forceinline__ void flag(const Params &P, int view, int bits) {
+    atomicOr(&P.status[view], bits);
+}
+
+__device__ __forceinline__ bool finite3f(float a, float b, float c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+__device__ __forceinline__ bool finite3d(const double *v) {
+    return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
+}
+
+// Workgroup -> (view, tile). Consecutive workgroup ids land on different XCDs
+// (round robin); the remap hands each XCD a contiguous run of tiles so that
+// tiles of one view share that XCD's L2.
+__device__ __forceinline__ bool tile_of_block(const Params &P, int &view, int &tile_y, int &tile_x,
+                                              int &logical) {
+    const int b = blockIdx.x;
+    logical = (b % kXcds) * P.chunk + b / kXcds;
+    if (logical >= P.total_tiles) return false;
+    view = logical / P.tiles_per_view;
+    const int r = logical - view * P.tiles_per_view;
+    tile_y = r / P.tiles_x;
+    tile_x = r - tile_y * P.tiles_x;
+    return true;
+}
+
+__device__ __forceinline__ void pixel_of_lane(const Params &P, int tile_y, int tile_x, int &row,
+                                              int &col) {
+    const int t = threadIdx.x;
+    if (P.bundle) {
+        row = 0;
+        col = tile_x * kBlock + t;
+    } else {
+        const int w = t >> 6, l = t & 63;
+        row = tile_y * kTile + (w >> 1) * 8 + (l >> 3);
+        col = tile_x * kTile + (w & 1) * 8 + (l & 7);
+    }
+}
+
+// ------------------------------------------------------------- ray setup --
+// kernels/projection.py:79-102 in FP64; same operation order as the oracle.
+
+struct World {
+    double wsr[3], wd[3], os[3], od[3];
+};
+
+__device__ __forceinline__ void ray_world(const Params &P, const double *pose, int view, int row,
+                                          int col, World &w) {
+    double src[3], end[3];
+    if (!P.bundle) {
+        const double *pn = P.panels + 12 * (size_t)view;
+        for (int i = 0; i < 3; ++i) {
+            src[i] = pn[i];
+            end[i] = pn[3 + i] + (double)col * pn[6 + i] + (double)row * pn[9 + i];
+        }
+    } else {
+        const size_t ray = (size_t)view * P.cols + col;
+        for (int i = 0; i < 3; ++i) {
+            src[i] = P.ray_src[3 * ray + i];
+            end[i] = P.ray_end[3 * ray + i];
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        w.wsr[i] = src[i] - pose[9 + i];
+        w.wd[i] = end[i] - src[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        w.os[i] = pose[i] * w.wsr[0] + pose[3 + i] * w.wsr[1] + pose[6 + i] * w.wsr[2];
+        w.od[i] = pose[i] * w.wd[0] + pose[3 + i] * w.wd[1] + pose[6 + i] * w.wd[2];
+    }
+}
+
+struct Ray {
+    double o[3], d[3], len;
+    bool valid;
+};
+
+__device__ __forceinline__ void ray_grid(const Params &P, const World &w, Ray &r) {
+    double rel[3];
+    for (int i = 0; i < 3; ++i) rel[i] = w.os[i] - P.go[i];
+    for (int i = 0; i < 3; ++i) {
+        r.o[i] = P.g[3 * i] * rel[0] + P.g[3 * i + 1] * rel[1] + P.g[3 * i + 2] * rel[2];
+        r.d[i] = P.g[3 * i] * w.od[0] + P.g[3 * i + 1] * w.od[1] + P.g[3 * i + 2] * w.od[2];
+    }
+    r.len = sqrt(w.wd[0] * w.wd[0] + w.wd[1] * w.wd[1] + w.wd[2] * w.wd[2]);
+    // kernels/projection.py:110-121 (valid_ray)
+    r.valid = finite3d(r.o) && finite3d(r.d) && finite3d(w.os) && finite3d(w.od) &&
+              finite3d(w.wsr) && finite3d(w.wd) && isfinite(r.len) && r.len > 0.0;
+}
+
+// --------------------------------------------------------- FP32 sampling --
+
+struct Interval32 {
+    float lower, upper;
+    int lo_axis, up_axis;    // active clip face (-1: none)
+    float lo_o, lo_d, up_o, up_d;
+};
+
+// kernels/projection_fp32.py:77-109; only the single active face derivative
+// is non-zero, so the interval keeps its axis and two scalars per end.
+__device__ __forceinline__ Interval32 interval32(const Params &P, const float *o, const float *d) {
+    Interval32 iv;
+    iv.lower = 0.0f;
+    iv.upper = 1.0f;
+    iv.lo_axis = iv.up_axis = -1;
+    iv.lo_o = iv.lo_d = iv.up_o = iv.up_d = 0.0f;
+    const float highs[3] = {P.hx, P.hy, P.hz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float low = -0.5f, high = highs[a], v = d[a];
+        if (v == 0.0f) {
+            if (o[a] < low || o[a] > high) iv.upper = -1.0f;
+        } else {
+            float first = (low - o[a]) / v, last = (high - o[a]) / v;
+            if (first > last) {
+                const float t = first;
+                first = last;
+                last = t;
+            }
+            if (first > iv.lower) {
+                iv.lower = first;
+                iv.lo_axis = a;
+                iv.lo_o = -1.0f / v;
+                iv.lo_d = -first / v;
+            }
+            if (last < iv.upper) {
+                iv.upper = last;
+                iv.up_axis = a;
+                iv.up_o = -1.0f / v;
+                iv.up_d = -last / v;
+            }
+        }
+    }
+    return iv;
+}
+
+template <typename T>
+__device__ __forceinline__ float load_value(const T *f, int i);
+template <>
+__device__ __forceinline__ float load_value<float>(const float *f, int i) {
+    return f[i];
+}
+template <>
+__device__ __forceinline__ float load_value<__hip_bfloat16>(const __hip_bfloat16 *f, int i) {
+    const uint16_t bits = reinterpret_cast<const uint16_t *>(f)[i];
+    return __uint_as_float((uint32_t)bits << 16);
+}
+template <>
+__device__ __forceinline__ float load_value<__half>(const __half *f, int i) {
+    return __half2float(f[i]);
+}
+
+struct Cell32 {
+    int base;                // linear index of the lower corner
+    int lx, ly, lz;          // lower corner
+    float fx, fy, fz;        // fractions
+    float slx, sly, slz;     // slopes (0/1)
+};
+
+// kernels/projection_fp32.py:64-74 + 130-140. Returns false outside support
+// (NaN and infinities fail the comparisons, as finite_point does).
+__device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, float p2, Cell32 &c) {
+    if (!(p0 >= -0.5f && p1 >= -0.5f && p2 >= -0.5f && p0 <= P.hx && p1 <= P.hy && p2 <= P.hz))
+        return false;
+    const float v0 = fminf(fmaxf(p0, 0.0f), P.cx);
+    const float v1 = fminf(fmaxf(p1, 0.0f), P.cy);
+    const float v2 = fminf(fmaxf(p2, 0.0f), P.cz);
+    c.lx = min((int)floorf(v0), P.capx);
+    c.ly = min((int)floorf(v1), P.capy);
+    c.lz = min((int)floorf(v2), P.capz);
+    c.fx = v0 - (float)c.lx;
+    c.fy = v1 - (float)c.ly;
+    c.fz = v2 - (float)c.lz;
+    c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
+    c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
+    c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
+    c.base = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+    return true;
+}
+
+// corner k value offsets: bit0 x, bit1 y, bit2 z (kernels/projection.py:217-222;
+// lower <= n-2 so only singleton axes clamp, which a zero stride reproduces).
+__device__ __forceinline__ int corner_offset(const Params &P, int k) {
+    return ((k & 1) ? P.sx : 0) + ((k & 2) ? P.sy : 0) + ((k & 4) ? P.sz : 0);
+}
+
+// corner weights (f0*f1)*f2 (kernels/projection_fp32.py:143-154)
+__device__ __forceinline__ void weights32(const Cell32 &c, float *w) {
+    const float ax = 1.0f - c.fx, ay = 1.0f - c.fy, az = 1.0f - c.fz;
+    const float xy[4] = {ax * ay, c.fx * ay, ax * c.fy, c.fx * c.fy};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = xy[k & 3] * ((k & 4) ? c.fz : az);
+}
+
+template <typename T>
+__device__ __forceinline__ void load8(const Params &P, const T *field, const Cell32 &c, float *v) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = load_value<T>(field, c.base + corner_offset(P, k));
+}
+
+// sum_k fma(w_k, v_k, .) in corner order (kernels/projection_fp32.py:165-168)
+__device__ __forceinline__ float interpolate32(const float *w, const float *v) {
+    float value = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) value = fmaf(w[k], v[k], value);
+    return value;
+}
+
+// kernels/projection_fp32.py:196-208 (differences before interpolation)
+__device__ __forceinline__ void slopes32(const Cell32 &c, const float *v, float *g) {
+    const float fx = c.fx, fy = c.fy, fz = c.fz;
+    const float ax = 1.0f - fx, ay = 1.0f - fy, az = 1.0f - fz;
+    const float x0 = fmaf(fy, v[3] - v[2], ay * (v[1] - v[0]));
+    const float x1 = fmaf(fy, v[7] - v[6], ay * (v[5] - v[4]));
+    g[0] = c.slx * fmaf(fz, x1, az * x0);
+    const float y0 = fmaf(fx, v[3] - v[1], ax * (v[2] - v[0]));
+    const float y1 = fmaf(fx, v[7] - v[5], ax * (v[6] - v[4]));
+    g[1] = c.sly * fmaf(fz, y1, az * y0);
+    const float z0 = fmaf(fx, v[5] - v[1], ax * (v[4] - v[0]));
+    const float z1 = fmaf(fx, v[7] - v[3], ax * (v[6] - v[2]));
+    g[2] = c.slz * fmaf(fy, z1, ay * z0);
+}
+
+// kernels/projection_fp32.py:240-252: RN(factor*weight), |x| < 2^-126 -> signed 0
+__device__ __forceinline__ float scatter_increment(float factor, float weight) {
+    const float x = factor * weight;
+    return fabsf(x) < 0x1p-126f ? copysignf(0.0f, x) : x;
+}
+
+// Compensated (Kahan) accumulator, used only by half-storage kernels.
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void init() { s = c = 0.0f; }
+    __device__ __forceinline__ void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+template <bool COMPENSATED>
+struct Acc {
+    float s;
+    __device__ __forceinline__ void init() { s = 0.0f; }
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ float value() const { return s; }
+};
+template <>
+struct Acc<true> {
+    Kahan k;
+    __device__ __forceinline__ void init() { k.init(); }
+    __device__ __forceinline__ void add(float x) { k.add(x); }
+    __device__ __forceinline__ float value() const { return k.s; }
+};
+
+// ------------------------------------------------------- FP32 forward ------
+// kernels/projection_fp32.py:334-374
+
+template <typename T, bool COMPENSATED>
+__global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__restrict__ field,
+                                                           float *__restrict__ out) {
+    int view, ty, tx, logical;
+    if (!tile_of_block(P, view, ty, tx, logical)) return;
+    int row, col;
+    pixel_of_lane(P, ty, tx, row, col);
+    if (row >= P.rows || col >= P.cols) return;
+    const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+    const double *pose = P.poses + 12 * (size_t)view;
+    World w;
+    ray_world(P, pose, view, row, col, w);
+    Ray r;
+    ray_grid(P, w, r);
+    float o[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
+    float d[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
+    const float len = (float)r.len;
+    if (!r.valid || !finite3f(o[0], o[1], o[2]) || !finite3f(d[0], d[1], d[2]) || !isfinite(len) ||
+        len <= 0.0f) {
+        flag(P, view, DPT_STATUS_RANGE);
+        out[ray] = 0.0f;
+        return;
+    }
+    const Interval32 iv = interval32(P, o, d);
+    Acc<COMPENSATED> total;
+    total.init();
+    float result = 0.0f;
+    if (iv.upper > iv.lower) {
+        const float step = (iv.upper - iv.lower) / (float)P.samples;
+        bool bad = false;
+        for (int k = 0; k < P.samples; ++k) {
+            const float pos = fmaf((float)k + 0.5f, step, iv.lower);
+            const float p0 = fmaf(pos, d[0], o[0]);
+            const float p1 = fmaf(pos, d[1], o[1]);
+            const float p2 = fmaf(pos, d[2], o[2]);
+            bad |= !finite3f(p0, p1, p2);
+            Cell32 c;
+            float value = 0.0f;
+            if (cell32(P, p0, p1, p2, c)) {
+                float wt[8], v[8];
+                weights32(c, wt);
+                load8<T>(P, field, c, v);
+                value = interpolate32(wt, v);
+            }
+            total.add(value);
+        }
+        if (bad) flag(P, view, DPT_STATUS_RANGE);
+        result = total.value() * (len * step);
+    }
+    out[ray] = result;
+    if (!isfinite(result)) flag(P, view, DPT_STATUS_RANGE);
+}
+
+// ----------------------------------------------------- pose reduction -----
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum C components over the 256 lanes in a fixed order; lane c < C of the
+// block writes partial[c].
+template <int C>
+__device__ __forceinline__ void block_store(const double *g, double *partial) {
+    __shared__ double red[kBlock / 64][kMaxComponents];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double s = wave_sum(g[c]);
+        if (l == 0) red[w][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < C) {
+        const int c = threadIdx.x;
+        partial[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+    }
+}
+
+// FP64 pose epilogue: kernels/projection_fp32.py:562-602 / projection.py:1156-1217
+template <int C>
+__device__ __forceinline__ void pose_epilogue(const Params &P, const double *pose, const World &w,
+                                              double factor, const double *og, const double *dg,
+                                              double *out) {
+    double sa[3], da[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double s = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+        const double t = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+        sa[i] = factor * s;
+        da[i] = factor * t;
+    }
+    if constexpr (C == 12) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[3 * r + c] = w.wsr[r] * sa[c] + w.wd[r] * da[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            out[9 + a] = -(pose[3 * a] * sa[0] + pose[3 * a + 1] * sa[1] + pose[3 * a + 2] * sa[2]);
+    } else {
+        const double *os = w.os, *od = w.od;
+        out[0] = -sa[0];
+        out[1] = -sa[1];
+        out[2] = -sa[2];
+        out[3] = (sa[1] * os[2] - sa[2] * os[1]) + (da[1] * od[2] - da[2] * od[1]);
+        out[4] = (sa[2] * os[0] - sa[0] * os[2]) + (da[2] * od[0] - da[0] * od[2]);
+        out[5] = (sa[0] * os[1] - sa[1] * os[0]) + (da[0] * od[1] - da[1] * od[0]);
+    }
+}
+
+// Register cell accumulator for the volume cotangent (reference Vec8 sums,
+// kernels/projection_fp32.py:261-269), flushed to the FP64 accumulator.
+struct CellSums {
+    int base;
+    float s[8];
+    __device__ __forceinline__ void reset() {
+        base = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] = 0.0f;
+    }
+    __device__ __forceinline__ void flush(const Params &P, double *acc) {
+        if (base < 0) return;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)s[k]);
+        reset();
+    }
+};
+
+// ---------------------------------------------------------- FP32 VJP ------
+// kernels/projection_fp32.py:377-617
+
+template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED>
+__global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
+                                                       const float *__restrict__ seeds,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ partials) {
+    int view, ty, tx, logical;
+    if (!tile_of_block(P, view, ty, tx, logical)) return;
+    int row, col;
+    pixel_of_lane(P, ty, tx, row, col);
+    double gradient[kMaxComponents];
+#pragma unroll
+    for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
+    const double *pose = P.poses + 12 * (size_t)view;
+    if (row < P.rows && col < P.cols) {
+        const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        float o[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
+        float d[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
+        const float len = (float)r.len;
+        if (!r.valid || !finite3f(o[0], o[1], o[2]) || !finite3f(d[0], d[1], d[2]) ||
+            !isfinite(len) || len <= 0.0f) {
+            flag(P, view, DPT_STATUS_RANGE);
+        } else {
+            const Interval32 iv = interval32(P, o, d);
+            const float span = iv.upper - iv.lower;
+            const float seed = seeds[ray];
+            if (!isfinite(seed)) flag(P, view, DPT_STATUS_RANGE);
+            if (span > 0.0f && seed != 0.0f) {
+                const float step = span / (float)P.samples;
+                const float physical_step = len * step;
+                const float volume_factor = seed * physical_step;
+                const bool bounded = isfinite(volume_factor) && fabsf(volume_factor) <= 1.0f;
+                Acc<COMPENSATED> mean, dor0, dor1, dor2, shl, shu;
+                float ddir[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
+                mean.init();
+                dor0.init();
+                dor1.init();
+                dor2.init();
+                shl.init();
+                shu.init();
+                CellSums cs;
+                cs.reset();
+                bool bad = false;
+                for (int k = 0; k < P.samples; ++k) {
+                    const float pos = fmaf((float)k + 0.5f, step, iv.lower);
+                    const float p0 = fmaf(pos, d[0], o[0]);
+                    const float p1 = fmaf(pos, d[1], o[1]);
+                    const float p2 = fmaf(pos, d[2], o[2]);
+                    bad |= !finite3f(p0, p1, p2);
+                    Cell32 c;
+                    const bool inside = cell32(P, p0, p1, p2, c);
+                    float wt[8];
+                    if (inside) weights32(c, wt);
+                    if constexpr (POSE) {
+                        float value = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+                        if (inside) {
+                            float v[8];
+                            load8<T>(P, field, c, v);
+                            value = interpolate32(wt, v);
+                            slopes32(c, v, g);
+                        }
+                        mean.add(value);
+                        dor0.add(g[0]);
+                        dor1.add(g[1]);
+                        dor2.add(g[2]);
+                        if constexpr (COMPENSATED) {
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                // compensated fma accumulation of pos * g
+                                const float y = fmaf(pos, g[a], -ddc[a]);
+                                const float t = ddir[a] + y;
+                                ddc[a] = (t - ddir[a]) - y;
+                                ddir[a] = t;
+                            }
+                        } else {
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) ddir[a] = fmaf(pos, g[a], ddir[a]);
+                        }
+                        const float alpha = ((float)k + 0.5f) / (float)P.samples;
+                        const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
+                        if constexpr (COMPENSATED) {
+                            shl.add((1.0f - alpha) * along);
+                            shu.add(alpha * along);
+                        } else {
+                            shl.s = fmaf(1.0f - alpha, along, shl.s);
+                            shu.s = fmaf(alpha, along, shu.s);
+                        }
+                    }
+                    if constexpr (VOL) {
+                        if (inside) {
+                            if (bounded) {
+                                if (c.base != cs.base) {
+                                    cs.flush(P, acc);
+                                    cs.base = c.base;
+                                }
+#pragma unroll
+                                for (int kk = 0; kk < 8; ++kk)
+                                    cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
+                            } else {
+#pragma unroll
+                                for (int kk = 0; kk < 8; ++kk) {
+                                    const float inc = scatter_increment(volume_factor, wt[kk]);
+                                    if (!isfinite(inc)) bad = true;
+                                    if (inc != 0.0f)
+                                        unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
+                                }
+                            }
+                        } else {
+                            cs.flush(P, acc);
+                        }
+                    }
+                }
+                if constexpr (VOL) cs.flush(P, acc);
+                if (bad) flag(P, view, DPT_STATUS_RANGE);
+                if constexpr (POSE) {
+                    const float m = mean.value();
+                    const float lower_seed = fmaf(span, shl.value(), -m);
+                    const float upper_seed = fmaf(span, shu.value(), m);
+                    const float dor[3] = {dor0.value(), dor1.value(), dor2.value()};
+                    double og[3], dg[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const float lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0f;
+                        const float lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0f;
+                        const float up_o = (iv.up_axis == a) ? iv.up_o : 0.0f;
+                        const float up_d = (iv.up_axis == a) ? iv.up_d : 0.0f;
+                        og[a] = (double)fmaf(upper_seed, up_o, fmaf(lower_seed, lo_o, span * dor[a]));
+                        dg[a] = (double)fmaf(upper_seed, up_d, fmaf(lower_seed, lo_d, span * ddir[a]));
+                    }
+                    const float factor = (seed * len) / (float)P.samples;
+                    pose_epilogue<C>(P, pose, w, (double)factor, og, dg, gradient);
+                }
+            }
+        }
+    }
+    if constexpr (POSE) block_store<C>(gradient, partials + (size_t)logical * C);
+}
+
+// --------------------------------------------------------- FP64 sampling --
+
+struct Interval64 {
+    double lower, upper;
+    int lo_axis, up_axis;
+    double lo_o, lo_d, up_o, up_d;
+};
+
+// kernels/projection.py:147-184
+__device__ __forceinline__ Interval64 interval64(const Params &P, const double *o, const double *d) {
+    Interval64 iv;
+    iv.lower = 0.0;
+    iv.upper = 1.0;
+    iv.lo_axis = iv.up_axis = -1;
+    iv.lo_o = iv.lo_d = iv.up_o = iv.up_d = 0.0;
+    const double highs[3] = {(double)P.nx - 0.5, (double)P.ny - 0.5, (double)P.nz - 0.5};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double low = -0.5, high = highs[a], v = d[a];
+        if (v == 0.0) {
+            if (o[a] < low || o[a] > high) iv.upper = -1.0;
+        } else {
+            double first = (low - o[a]) / v, last = (high - o[a]) / v;
+            if (first > last) {
+                const double t = first;
+                first = last;
+                last = t;
+            }
+            if (first > iv.lower) {
+                iv.lower = first;
+                iv.lo_axis = a;
+                iv.lo_o = -1.0 / v;
+                iv.lo_d = -first / v;
+            }
+            if (last < iv.upper) {
+                iv.upper = last;
+                iv.up_axis = a;
+                iv.up_o = -1.0 / v;
+                iv.up_d = -last / v;
+            }
+        }
+    }
+    return iv;
+}
+
+struct Cell64 {
+    int base;
+    double fx, fy, fz;
+    double slx, sly, slz;
+};
+
+// kernels/projection.py:124-134 + 199-214
+__device__ __forceinline__ bool cell64(const Params &P, const double *p, Cell64 &c) {
+    const double hx = (double)P.nx - 0.5, hy = (double)P.ny - 0.5, hz = (double)P.nz - 0.5;
+    if (!(p[0] >= -0.5 && p[1] >= -0.5 && p[2] >= -0.5 && p[0] <= hx && p[1] <= hy && p[2] <= hz))
+        return false;
+    const double mx = (double)(P.nx - 1), my = (double)(P.ny - 1), mz = (double)(P.nz - 1);
+    const double v0 = fmin(fmax(p[0], 0.0), mx);
+    const double v1 = fmin(fmax(p[1], 0.0), my);
+    const double v2 = fmin(fmax(p[2], 0.0), mz);
+    const int lx = min((int)floor(v0), P.capx);
+    const int ly = min((int)floor(v1), P.capy);
+    const int lz = min((int)floor(v2), P.capz);
+    c.fx = v0 - (double)lx;
+    c.fy = v1 - (double)ly;
+    c.fz = v2 - (double)lz;
+    c.slx = (p[0] >= 0.0 && p[0] < mx) ? 1.0 : 0.0;
+    c.sly = (p[1] >= 0.0 && p[1] < my) ? 1.0 : 0.0;
+    c.slz = (p[2] >= 0.0 && p[2] < mz) ? 1.0 : 0.0;
+    c.base = (lz * P.ny + ly) * P.nx + lx;
+    return true;
+}
+
+// kernels/projection.py:236-274 (weighted lerps; differences before interpolation)
+template <bool GRAD>
+__device__ __forceinline__ double sample_cell64(const Cell64 &c, const double *v, double *g) {
+    const double v000 = v[0], v100 = v[1], v010 = v[2], v110 = v[3];
+    const double v001 = v[4], v101 = v[5], v011 = v[6], v111 = v[7];
+    const double fx = c.fx, fy = c.fy, fz = c.fz;
+    const double ax = 1.0 - fx, ay = 1.0 - fy, az = 1.0 - fz;
+    const double x00 = ax * v000 + fx * v100, x10 = ax * v010 + fx * v110;
+    const double x01 = ax * v001 + fx * v101, x11 = ax * v011 + fx * v111;
+    const double xy0 = ay * x00 + fy * x10, xy1 = ay * x01 + fy * x11;
+    const double value = az * xy0 + fz * xy1;
+    if constexpr (GRAD) {
+        const double dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
+        const double dx0 = ay * dx00 + fy * dx10, dx1 = ay * dx01 + fy * dx11;
+        const double dy0 = ax * (v010 - v000) + fx * (v110 - v100);
+        const double dy1 = ax * (v011 - v001) + fx * (v111 - v101);
+        const double dz0 = ax * (v001 - v000) + fx * (v101 - v100);
+        const double dz1 = ax * (v011 - v010) + fx * (v111 - v110);
+        g[0] = c.slx * (az * dx0 + fz * dx1);
+        g[1] = c.sly * (az * dy0 + fz * dy1);
+        g[2] = c.slz * (ay * dz0 + fy * dz1);
+    }
+    return value;
+}
+
+template <typename T>
+__device__ __forceinline__ void load8d(const Params &P, const T *field, int base, double *v) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (double)load_value<T>(field, base + corner_offset(P, k));
+}
+
+// kernels/projection.py:802-846 (midpoint, FP64 sampling)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void forward64_kernel(Params P, const T *__restrict__ field,
+                                                           double *__restrict__ out) {
+    int view, ty, tx, logical;
+    if (!tile_of_block(P, view, ty, tx, logical)) return;
+    int row, col;
+    pixel_of_lane(P, ty, tx, row, col);
+    if (row >= P.rows || col >= P.cols) return;
+    const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+    const double *pose = P.poses + 12 * (size_t)view;
+    World w;
+    ray_world(P, pose, view, row, col, w);
+    Ray r;
+    ray_grid(P, w, r);
+    if (!r.valid) {
+        flag(P, view, DPT_STATUS_RANGE);
+        out[ray] = 0.0;
+        return;
+    }
+    const Interval64 iv = interval64(P, r.o, r.d);
+    double integral = 0.0;
+    if (iv.upper > iv.lower) {
+        const double step = (iv.upper - iv.lower) / (double)P.samples;
+        bool bad = false;
+        for (int k = 0; k < P.samples; ++k) {
+            const double pos = iv.lower + ((double)k + 0.5) * step;
+            const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1], r.o[2] + pos * r.d[2]};
+            bad |= !finite3d(p);
+            Cell64 c;
+            if (cell64(P, p, c)) {
+                double v[8];
+                load8d<T>(P, field, c.base, v);
+                integral += sample_cell64<false>(c, v, nullptr);
+            }
+        }
+        if (bad) flag(P, view, DPT_STATUS_RANGE);
+        integral *= r.len * step;
+    }
+    out[ray] = integral;
+    if (!isfinite(integral)) flag(P, view, DPT_STATUS_RANGE);
+}
+
+// kernels/numerics.py:11-56: product through a mantissa/exponent split
+__device__ __forceinline__ double product4(double a, double b, double c, double d) {
+    const double f[4] = {a, b, c, d};
+    double ordinary = a * b * c * d;
+    bool invalid = false, zero = false;
+    for (int i = 0; i < 4; ++i) {
+        invalid |= !isfinite(f[i]);
+        zero |= f[i] == 0.0;
+    }
+    if (invalid) return ordinary;
+    if (zero) return 0.0;
+    double m = 1.0;
+    int e = 0;
+    for (int i = 0; i < 4; ++i) {
+        int ei;
+        const double mi = frexp(f[i], &ei);
+        m = (i == 0) ? mi : m * mi;
+        e += ei;
+    }
+    return ldexp(m, e);
+}
+
+// kernels/projection.py:873-1237 (midpoint, FP64 sampling, FP32 increments)
+template <typename T, bool VOL, bool POSE, int C>
+__global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__restrict__ field,
+                                                       const double *__restrict__ seeds,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ partials) {
+    int view, ty, tx, logical;
+    if (!tile_of_block(P, view, ty, tx, logical)) return;
+    int row, col;
+    pixel_of_lane(P, ty, tx, row, col);
+    double gradient[kMaxComponents];
+#pragma unroll
+    for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
+    const double *pose = P.poses + 12 * (size_t)view;
+    if (row < P.rows && col < P.cols) {
+        const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        if (!r.valid) {
+            flag(P, view, DPT_STATUS_RANGE);
+        } else {
+            const Interval64 iv = interval64(P, r.o, r.d);
+            double span = iv.upper - iv.lower;
+            const double seed = seeds[ray];
+            if (seed == 0.0) span = 0.0;
+            if (span > 0.0) {
+                double factor = seed * r.len;
+                const double S = (double)P.samples;
+                const double step = span / S;
+                double volume_factor = factor * span / S;
+                if (!isfinite(factor)) volume_factor = product4(seed, r.len, span, 1.0 / S);
+                const bool pow2 = (P.samples & (P.samples - 1)) == 0;
+                const double inverse_samples = pow2 ? 1.0 / S : 0.0;
+                double mean = 0.0, dor[3] = {0, 0, 0}, ddir[3] = {0, 0, 0};
+                double shift_lower = 0.0, shift_upper = 0.0;
+                bool bad = false;
+                for (int k = 0; k < P.samples; ++k) {
+                    const double alpha = pow2 ? ((double)k + 0.5) * inverse_samples : ((double)k + 0.5) / S;
+                    const double pos = iv.lower + ((double)k + 0.5) * step;
+                    const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1],
+                                         r.o[2] + pos * r.d[2]};
+                    bad |= !finite3d(p);
+                    Cell64 c;
+                    if (!cell64(P, p, c)) continue;
+                    if constexpr (POSE) {
+                        double v[8], g[3];
+                        load8d<T>(P, field, c.base, v);
+                        const double value = sample_cell64<true>(c, v, g);
+                        mean += value;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            dor[a] += g[a];
+                            ddir[a] += pos * g[a];
+                        }
+                        const double along = g[0] * r.d[0] + g[1] * r.d[1] + g[2] * r.d[2];
+                        shift_lower += (1.0 - alpha) * along;
+                        shift_upper += alpha * along;
+                    }
+                    if constexpr (VOL) {
+                        // kernels/projection.py:396-430: seed * weights in axis order, one FP32 rounding
+#pragma unroll
+                        for (int kk = 0; kk < 8; ++kk) {
+                            double contribution = volume_factor;
+                            contribution *= (kk & 1) ? c.fx : 1.0 - c.fx;
+                            contribution *= (kk & 2) ? c.fy : 1.0 - c.fy;
+                            contribution *= (kk & 4) ? c.fz : 1.0 - c.fz;
+                            const float rounded = (float)contribution;
+                            if (!isfinite(rounded)) bad = true;
+                            if (rounded != 0.0f)
+                                unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)rounded);
+                        }
+                    }
+                }
+                if (bad) flag(P, view, DPT_STATUS_RANGE);
+                if constexpr (POSE) {
+                    const double lower_seed = -mean + span * shift_lower;
+                    const double upper_seed = mean + span * shift_upper;
+                    double og[3], dg[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0;
+                        const double lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
+                        const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0;
+                        const double up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
+                        og[a] = span * dor[a] + lower_seed * lo_o + upper_seed * up_o;
+                        dg[a] = span * ddir[a] + lower_seed * lo_d + upper_seed * up_d;
+                    }
+                    factor /= S;
+                    if (isfinite(factor)) {
+                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient);
+                    } else {
+                        // kernels/projection.py:1163-1183 range-safe products
+                        double sa[3], da[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const double s = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+                            const double t = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+                            sa[i] = product4(seed, r.len, 1.0 / S, s);
+                            da[i] = product4(seed, r.len, 1.0 / S, t);
+                        }
+                        Params unit = P;
+                        for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
+                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (POSE) block_store<C>(gradient, partials + (size_t)logical * C);
+}
+
+// ------------------------------------------------------- small kernels ----
+
+// Fixed-order FP64 sum of the tile partials of one view (kernels/reductions.py
+// role), then write or accumulate and flag non-finite results.
+template <int C>
+__global__ __launch_bounds__(kBlock) void pose_reduce_kernel(const double *__restrict__ partials,
+                                                             int tiles_per_view, double *grad_pose,
+                                                             int accumulate, int *status) {
+    const int view = blockIdx.x;
+    __shared__ double s[kBlock];
+    const double *base = partials + (size_t)view * tiles_per_view * C;
+    for (int c = 0; c < C; ++c) {
+        double v = 0.0;
+        for (int t = threadIdx.x; t < tiles_per_view; t += kBlock) v += base[(size_t)t * C + c];
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+            if (threadIdx.x < stride) s[threadIdx.x] += s[threadIdx.x + stride];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double result = s[0];
+            if (accumulate) result += grad_pose[(size_t)view * C + c];
+            grad_pose[(size_t)view * C + c] = result;
+            if (!isfinite(result)) atomicOr(&status[view], DPT_STATUS_RANGE);
+        }
+        __syncthreads();
+    }
+}
+
+// kernels/projection_checks.py:20-24 / :36-46 (physical field: finite, >= 0)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void check_field_kernel(const T *__restrict__ field, int64_t n,
+                                                             int *status) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const float v = load_value<T>(field, (int)i);
+        bad |= !(isfinite(v) && v >= 0.0f);
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&status[0], DPT_STATUS_DOMAIN);
+}
+
+// kernels/projection_checks.py:49-58
+template <typename T>
+__global__ __launch_bounds__(kBlock) void check_finite_kernel(const T *__restrict__ x, int64_t n,
+                                                              int *status) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        bad |= !isfinite(x[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&status[0], DPT_STATUS_RANGE);
+}
+
+// kernels/batched_geometry.py:71-88 + kernels/geometry.py:55-74
+__global__ void check_poses_kernel(const double *__restrict__ poses, int views, int *status) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= views) return;
+    const double *R = poses + 12 * (size_t)v;
+    bool valid = isfinite(R[9]) && isfinite(R[10]) && isfinite(R[11]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            if (!isfinite(R[3 * i + j])) valid = false;
+            // (R^T R)[i][j] = sum_k R[k][i] R[k][j]
+            const double prod = R[i] * R[j] + R[3 + i] * R[3 + j] + R[6 + i] * R[6 + j];
+            const double expected = (i == j) ? 1.0 : 0.0;
+            if (!(fabs(prod - expected) <= 1e-10)) valid = false;
+        }
+    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                       R[2] * (R[3] * R[7] - R[4] * R[6]);
+    if (det < 0.0) valid = false;
+    if (!valid) atomicOr(&status[v], DPT_STATUS_POSE);
+}
+
+__global__ __launch_bounds__(kBlock) void fill_f64_kernel(double *__restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] = 0.0;
+}
+
+__global__ void fill_i32_kernel(int *__restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
+// FP64 accumulator -> FP32 gradient (write or accumulate) + finite check
+// (kernels/projection_checks.py:49-58 applied to the final gradient).
+__global__ __launch_bounds__(kBlock) void finish_volume_kernel(const double *__restrict__ acc,
+                                                               float *__restrict__ grad, int64_t n,
+                                                               int accumulate, int *status) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        float v = (float)acc[i];
+        if (accumulate) v += grad[i];
+        grad[i] = v;
+        bad |= !isfinite(v);
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&status[0], DPT_STATUS_RANGE);
+}
+
+// FP32 -> bf16/f16 with round-to-nearest-even, plus the physical-field check.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void convert_kernel(const float *__restrict__ src, T *__restrict__ dst,
+                                                         int64_t n, int *status) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const float v = src[i];
+        bad |= !(isfinite(v) && v >= 0.0f);
+        if constexpr (std::is_same<T, __half>::value) {
+            dst[i] = __float2half_rn(v);
+        } else {
+            dst[i] = __float2bfloat16(v);
+        }
+    }
+    if (status != nullptr && __any(bad) && (threadIdx.x & 63) == 0) atomicOr(&status[0], DPT_STATUS_DOMAIN);
+}
+
+// Count rays whose FP32 clipped interval is non-empty (effective samples =
+// count * samples); same setup arithmetic as the forward kernel.
+__global__ __launch_bounds__(kBlock) void count_hits_kernel(Params P, unsigned long long *count) {
+    int view, ty, tx, logical;
+    if (!tile_of_block(P, view, ty, tx, logical)) return;
+    int row, col;
+    pixel_of_lane(P, ty, tx, row, col);
+    bool hit = false;
+    if (row < P.rows && col < P.cols) {
+        const double *pose = P.poses + 12 * (size_t)view;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        float o[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
+        float d[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
+        if (r.valid) {
+            const Interval32 iv = interval32(P, o, d);
+            hit = iv.upper > iv.lower;
+        }
+    }
+    const unsigned long long lanes = __popcll(__ballot(hit));
+    if ((threadIdx.x & 63) == 0 && lanes) atomicAdd(count, lanes);
+}
+
+}  // namespace dpt
+
+// ================================================================ C ABI ====
+
+using namespace dpt;
+
+struct dpt_plan {
+    int device = 0;
+    Params base{};
+    int arithmetic = 0, storage = 0;
+    unsigned needs = 0;
+    int64_t voxels = 0, pixels = 0;
+    double *d_panels = nullptr, *d_ray_src = nullptr, *d_ray_end = nullptr;
+    int *d_status = nullptr;
+    int32_t *h_status = nullptr;
+    double *d_partials = nullptr;
+    double *d_acc = nullptr;
+    unsigned long long *d_count = nullptr;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed[2] = {false, false};
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(DPT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+struct DeviceGuard {
+    int previous = -1;
+    explicit DeviceGuard(int device) {
+        if (hipGetDevice(&previous) != hipSuccess) previous = -1;
+        if (previous != device) (void)hipSetDevice(device);
+    }
+    ~DeviceGuard() {
+        int now = -1;
+        if (previous >= 0 && hipGetDevice(&now) == hipSuccess && now != previous) (void)hipSetDevice(previous);
+    }
+};
+
+unsigned grid_blocks(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (unsigned)b;
+}
+
+Params launch_params(const dpt_plan *p, const double *poses) {
+    Params P = p->base;
+    P.poses = poses;
+    P.status = p->d_status;
+    return P;
+}
+
+unsigned tile_grid(const Params &P) { return (unsigned)(P.chunk * kXcds); }
+
+template <typename T>
+int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, hipStream_t s) {
+    const T *f = static_cast<const T *>(field);
+    if (p->arithmetic == DPT_ARITH_FP32) {
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL((forward32_kernel<T, false>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
+                               static_cast<float *>(out));
+        else
+            hipLaunchKernelGGL((forward32_kernel<T, true>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
+                               static_cast<float *>(out));
+    } else {
+        hipLaunchKernelGGL((forward64_kernel<T>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
+                           static_cast<double *>(out));
+    }
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+template <typename T, int C>
+int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
+                 hipStream_t s) {
+    const T *f = static_cast<const T *>(field);
+    const dim3 grid(tile_grid(P)), block(kBlock);
+    constexpr bool HALF = !std::is_same<T, float>::value;
+    if (p->arithmetic == DPT_ARITH_FP32) {
+        const float *sd = static_cast<const float *>(seeds);
+        if (vol && pose)
+            hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else if (vol)
+            hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else
+            hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+    } else {
+        const double *sd = static_cast<const double *>(seeds);
+        if (vol && pose)
+            hipLaunchKernelGGL((vjp64_kernel<T, true, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else if (vol)
+            hipLaunchKernelGGL((vjp64_kernel<T, true, false, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else
+            hipLaunchKernelGGL((vjp64_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+    }
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+template <typename T>
+int launch_vjp(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
+               int components, hipStream_t s) {
+    if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, s);
+    return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, s);
+}
+
+void timer_begin(dpt_plan *p, int kind, hipStream_t s) {
+    if (p->timing) (void)hipEventRecord(p->ev[2 * kind], s);
+}
+void timer_end(dpt_plan *p, int kind, hipStream_t s) {
+    if (p->timing) {
+        (void)hipEventRecord(p->ev[2 * kind + 1], s);
+        p->timed[kind] = true;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dpt_abi_version(void) { return DPT_ABI_VERSION; }
+
+const char *dpt_last_error(void) { return g_error.c_str(); }
+
+int dpt_plan_destroy(dpt_plan *p) {
+    if (p == nullptr) return DPT_OK;
+    DeviceGuard guard(p->device);
+    (void)hipFree(p->d_panels);
+    (void)hipFree(p->d_ray_src);
+    (void)hipFree(p->d_ray_end);
+    (void)hipFree(p->d_status);
+    (void)hipFree(p->d_partials);
+    (void)hipFree(p->d_acc);
+    (void)hipFree(p->d_count);
+    if (p->h_status) (void)hipHostFree(p->h_status);
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete p;
+    return DPT_OK;
+}
+
+int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols, const double *panels,
+                    const double *ray_sources, const double *ray_endpoints, int32_t samples, int32_t arithmetic,
+                    int32_t storage, uint32_t needs, int32_t device, dpt_plan **out) {
+    if (out == nullptr || grid == nullptr) return fail(DPT_E_INVALID, "grid and out are required");
+    *out = nullptr;
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
+        return fail(DPT_E_INVALID, "grid counts must be positive");
+    const int64_t voxels = (int64_t)grid->nx * grid->ny * grid->nz;
+    if (voxels > INT32_MAX) return fail(DPT_E_INVALID, "grid exceeds signed 32-bit indexing");
+    if (views < 1 || rows < 1 || cols < 1) return fail(DPT_E_INVALID, "views, rows and cols must be positive");
+    const int64_t pixels = (int64_t)views * rows * cols;
+    if (pixels > INT32_MAX) return fail(DPT_E_INVALID, "image exceeds signed 32-bit indexing");
+    if (samples < 1) return fail(DPT_E_INVALID, "samples must be positive");
+    if (arithmetic != DPT_ARITH_FP32 && arithmetic != DPT_ARITH_FP64)
+        return fail(DPT_E_INVALID, "arithmetic must be DPT_ARITH_FP32 or DPT_ARITH_FP64");
+    if (storage != DPT_STORE_F32 && storage != DPT_STORE_BF16 && storage != DPT_STORE_F16)
+        return fail(DPT_E_INVALID, "storage must be DPT_STORE_F32, DPT_STORE_BF16 or DPT_STORE_F16");
+    if (arithmetic == DPT_ARITH_FP32 && (grid->nx > (1 << 24) || grid->ny > (1 << 24) || grid->nz > (1 << 24)))
+        return fail(DPT_E_INVALID, "FP32 sampling requires grid axis counts at most 2**24");
+    if (panels == nullptr && (ray_sources == nullptr || ray_endpoints == nullptr))
+        return fail(DPT_E_INVALID, "supply panels or explicit ray sources and endpoints");
+    if (panels == nullptr && rows != 1) return fail(DPT_E_INVALID, "explicit rays require rows == 1");
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(grid->object_to_grid[i])) return fail(DPT_E_INVALID, "object_to_grid must be finite");
+
+    DeviceGuard guard(device);
+    dpt_plan *p = new dpt_plan();
+    p->device = device;
+    p->arithmetic = arithmetic;
+    p->storage = storage;
+    p->needs = needs;
+    p->voxels = voxels;
+    p->pixels = pixels;
+    Params &P = p->base;
+    P.nx = grid->nx;
+    P.ny = grid->ny;
+    P.nz = grid->nz;
+    P.sx = grid->nx > 1 ? 1 : 0;
+    P.sy = grid->ny > 1 ? grid->nx : 0;
+    P.sz = grid->nz > 1 ? grid->nx * grid->ny : 0;
+    P.hx = (float)grid->nx - 0.5f;
+    P.hy = (float)grid->ny - 0.5f;
+    P.hz = (float)grid->nz - 0.5f;
+    P.cx = (float)(grid->nx - 1);
+    P.cy = (float)(grid->ny - 1);
+    P.cz = (float)(grid->nz - 1);
+    P.capx = grid->nx - 2 > 0 ? grid->nx - 2 : 0;
+    P.capy = grid->ny - 2 > 0 ? grid->ny - 2 : 0;
+    P.capz = grid->nz - 2 > 0 ? grid->nz - 2 : 0;
+    std::memcpy(P.g, grid->object_to_grid, sizeof P.g);
+    std::memcpy(P.go, grid->grid_origin, sizeof P.go);
+    P.views = views;
+    P.rows = rows;
+    P.cols = cols;
+    P.samples = samples;
+    P.bundle = panels == nullptr ? 1 : 0;
+    if (P.bundle) {
+        P.tile_w = kBlock;
+        P.tile_h = 1;
+    } else {
+        P.tile_w = kTile;
+        P.tile_h = kTile;
+    }
+    P.tiles_x = (cols + P.tile_w - 1) / P.tile_w;
+    P.tiles_y = (rows + P.tile_h - 1) / P.tile_h;
+    P.tiles_per_view = P.tiles_x * P.tiles_y;
+    P.total_tiles = P.tiles_per_view * views;
+    P.chunk = (P.total_tiles + kXcds - 1) / kXcds;
+
+    auto cleanup = [&](int code, const std::string &msg) {
+        dpt_plan_destroy(p);
+        return fail(code, msg);
+    };
+#define ALLOC(ptr, bytes)                                                                   \
+    do {                                                                                    \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                              \
+            return cleanup(DPT_E_MEMORY, "hipMalloc failed for " #ptr);                     \
+    } while (0)
+
+    if (!P.bundle) {
+        ALLOC(p->d_panels, sizeof(double) * 12 * views);
+        if (hipMemcpy(p->d_panels, panels, sizeof(double) * 12 * views, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(DPT_E_DEVICE, "copy of panels failed");
+    } else {
+        const size_t bytes = sizeof(double) * 3 * (size_t)pixels;
+        ALLOC(p->d_ray_src, bytes);
+        ALLOC(p->d_ray_end, bytes);
+        if (hipMemcpy(p->d_ray_src, ray_sources, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->d_ray_end, ray_endpoints, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(DPT_E_DEVICE, "copy of rays failed");
+    }
+    P.panels = p->d_panels;
+    P.ray_src = p->d_ray_src;
+    P.ray_end = p->d_ray_end;
+    ALLOC(p->d_status, sizeof(int) * views);
+    if (hipMemset(p->d_status, 0, sizeof(int) * views) != hipSuccess)
+        return cleanup(DPT_E_DEVICE, "status clear failed");
+    if (hipHostMalloc((void **)&p->h_status, sizeof(int32_t) * views, hipHostMallocDefault) != hipSuccess)
+        return cleanup(DPT_E_MEMORY, "pinned status allocation failed");
+    if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.total_tiles);
+    if (needs & DPT_VOLUME) ALLOC(p->d_acc, sizeof(double) * (size_t)voxels);
+    ALLOC(p->d_count, sizeof(unsigned long long));
+    for (auto &e : p->ev)
+        if (hipEventCreate(&e) != hipSuccess) return cleanup(DPT_E_DEVICE, "event creation failed");
+#undef ALLOC
+    *out = p;
+    return DPT_OK;
+}
+
+int dpt_forward(dpt_plan *p, const void *field, const double *poses, void *out, void *stream) {
+    if (p == nullptr || field == nullptr || poses == nullptr || out == nullptr)
+        return fail(DPT_E_INVALID, "plan, field, poses and out are required");
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Params P = launch_params(p, poses);
+    switch (p->storage) {
+    case DPT_STORE_F32:
+        hipLaunchKernelGGL(check_field_kernel<float>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                           static_cast<const float *>(field), p->voxels, p->d_status);
+        break;
+    case DPT_STORE_BF16:
+        hipLaunchKernelGGL(check_field_kernel<__hip_bfloat16>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                           static_cast<const __hip_bfloat16 *>(field), p->voxels, p->d_status);
+        break;
+    default:
+        hipLaunchKernelGGL(check_field_kernel<__half>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                           static_cast<const __half *>(field), p->voxels, p->d_status);
+    }
+    hipLaunchKernelGGL(check_poses_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, poses, P.views, p->d_status);
+    HIP_TRY(hipGetLastError());
+    timer_begin(p, 0, s);
+    int rc;
+    if (p->storage == DPT_STORE_F32)
+        rc = launch_forward<float>(p, P, field, out, s);
+    else if (p->storage == DPT_STORE_BF16)
+        rc = launch_forward<__hip_bfloat16>(p, P, field, out, s);
+    else
+        rc = launch_forward<__half>(p, P, field, out, s);
+    timer_end(p, 0, s);
+    return rc;
+}
+
+int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, float *grad_field,
+            double *grad_pose, uint32_t flags, void *stream) {
+    if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr)
+        return fail(DPT_E_INVALID, "plan, field, poses and cotangent are required");
+    const bool vol = flags & DPT_VOLUME, pose = flags & DPT_POSE;
+    if (!vol && !pose) return fail(DPT_E_INVALID, "request at least one projection gradient");
+    if (vol && (grad_field == nullptr || !(p->needs & DPT_VOLUME)))
+        return fail(DPT_E_INVALID, "volume gradient requested without a destination or plan support");
+    if (pose && (grad_pose == nullptr || !(p->needs & DPT_POSE)))
+        return fail(DPT_E_INVALID, "pose gradient requested without a destination or plan support");
+    const int components = (flags & DPT_POSE_TANGENT) ? 6 : 12;
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Params P = launch_params(p, poses);
+    if (!(flags & DPT_NO_CHECKS)) {
+        switch (p->storage) {
+        case DPT_STORE_F32:
+            hipLaunchKernelGGL(check_field_kernel<float>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                               static_cast<const float *>(field), p->voxels, p->d_status);
+            break;
+        case DPT_STORE_BF16:
+            hipLaunchKernelGGL(check_field_kernel<__hip_bfloat16>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0,
+                               s, static_cast<const __hip_bfloat16 *>(field), p->voxels, p->d_status);
+            break;
+        default:
+            hipLaunchKernelGGL(check_field_kernel<__half>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                               static_cast<const __half *>(field), p->voxels, p->d_status);
+        }
+        hipLaunchKernelGGL(check_poses_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, poses, P.views,
+                           p->d_status);
+        if (p->arithmetic == DPT_ARITH_FP32)
+            hipLaunchKernelGGL(check_finite_kernel<float>, dim3(grid_blocks(p->pixels)), dim3(kBlock), 0, s,
+                               static_cast<const float *>(cotangent), p->pixels, p->d_status);
+        else
+            hipLaunchKernelGGL(check_finite_kernel<double>, dim3(grid_blocks(p->pixels)), dim3(kBlock), 0, s,
+                               static_cast<const double *>(cotangent), p->pixels, p->d_status);
+    }
+    if (vol)
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc, p->voxels);
+    HIP_TRY(hipGetLastError());
+    timer_begin(p, 1, s);
+    int rc;
+    if (p->storage == DPT_STORE_F32)
+        rc = launch_vjp<float>(p, P, field, cotangent, vol, pose, components, s);
+    else if (p->storage == DPT_STORE_BF16)
+        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, vol, pose, components, s);
+    else
+        rc = launch_vjp<__half>(p, P, field, cotangent, vol, pose, components, s);
+    timer_end(p, 1, s);
+    if (rc != DPT_OK) return rc;
+    const int accumulate = (flags & DPT_ACCUMULATE) ? 1 : 0;
+    if (pose) {
+        if (components == 6)
+            hipLaunchKernelGGL(pose_reduce_kernel<6>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
+                               P.tiles_per_view, grad_pose, accumulate, p->d_status);
+        else
+            hipLaunchKernelGGL(pose_reduce_kernel<12>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
+                               P.tiles_per_view, grad_pose, accumulate, p->d_status);
+    }
+    if (vol)
+        hipLaunchKernelGGL(finish_volume_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc,
+                           grad_field, p->voxels, accumulate, p->d_status);
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+int dpt_status(dpt_plan *p, int32_t *host_codes, int32_t clear, void *stream) {
+    if (p == nullptr || host_codes == nullptr) return fail(DPT_E_INVALID, "plan and host_codes are required");
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int views = p->base.views;
+    HIP_TRY(hipMemcpyAsync(p->h_status, p->d_status, sizeof(int32_t) * views, hipMemcpyDeviceToHost, s));
+    if (clear) hipLaunchKernelGGL(fill_i32_kernel, dim3((views + 255) / 256), dim3(256), 0, s, p->d_status, views);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(host_codes, p->h_status, sizeof(int32_t) * views);
+    return DPT_OK;
+}
+
+int dpt_convert_field(const float *src, void *dst, int64_t count, int32_t storage, int32_t *status, void *stream) {
+    if (src == nullptr || dst == nullptr || count < 0) return fail(DPT_E_INVALID, "src, dst and count are required");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (storage == DPT_STORE_BF16)
+        hipLaunchKernelGGL(convert_kernel<__hip_bfloat16>, dim3(grid_blocks(count)), dim3(kBlock), 0, s, src,
+                           static_cast<__hip_bfloat16 *>(dst), count, status);
+    else if (storage == DPT_STORE_F16)
+        hipLaunchKernelGGL(convert_kernel<__half>, dim3(grid_blocks(count)), dim3(kBlock), 0, s, src,
+                           static_cast<__half *>(dst), count, status);
+    else
+        return fail(DPT_E_INVALID, "storage must be DPT_STORE_BF16 or DPT_STORE_F16");
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+int dpt_set_timing(dpt_plan *p, int32_t enabled) {
+    if (p == nullptr) return fail(DPT_E_INVALID, "plan is required");
+    p->timing = enabled != 0;
+    p->timed[0] = p->timed[1] = false;
+    return DPT_OK;
+}
+
+double dpt_kernel_ms(dpt_plan *p, int32_t kind) {
+    if (p == nullptr || kind < 0 || kind > 1 || !p->timed[kind]) return -1.0;
+    DeviceGuard guard(p->device);
+    if (hipEventSynchronize(p->ev[2 * kind + 1]) != hipSuccess) return -1.0;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, p->ev[2 * kind], p->ev[2 * kind + 1]) != hipSuccess) return -1.0;
+    return (double)ms;
+}
+
+int64_t dpt_count_hit_rays(dpt_plan *p, const double *poses, void *stream) {
+    if (p == nullptr || poses == nullptr) {
+        fail(DPT_E_INVALID, "plan and poses are required");
+        return DPT_E_INVALID;
+    }
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Params P = launch_params(p, poses);
+    unsigned long long zero = 0, host = 0;
+    if (hipMemcpyAsync(p->d_count, &zero, sizeof zero, hipMemcpyHostToDevice, s) != hipSuccess) return DPT_E_DEVICE;
+    hipLaunchKernelGGL(count_hits_kernel, dim3(tile_grid(P)), dim3(kBlock), 0, s, P, p->d_count);
+    if (hipMemcpyAsync(&host, p->d_count, sizeof host, hipMemcpyDeviceToHost, s) != hipSuccess) return DPT_E_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return DPT_E_DEVICE;
+    return (int64_t)host;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+"""Host operator layer: configuration encoding, the plan cache and checkpoints.
+
+Replaces the reference's per-view workspace stack
+(python/dpt/projection.py:212-624 and python/dpt/torch/_projection_ops.py:
+35-175): one native multiview plan per (configuration, device, stream, active
+gradients), cached LRU-8 and guarded by a per-entry lock; one status readback
+per composed operation (python/dpt/_runtime.py:317-445).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import threading
+from collections import OrderedDict
+from dataclasses import dataclass, field
+from functools import lru_cache
+
+import numpy as np
+
+from . import _native
+from .contracts import ContractError, integer, raise_view_statuses
+from .geometry import BatchedGeometry, DetectorGeometry, RayBundleGeometry
+from .volumes import GridSpec
+
+CACHE_LIMIT = 8
+PRECISIONS = ("float32", "float64")
+STORAGES = ("float32", "bfloat16", "float16")
+
+
+@dataclass(frozen=True)
+class ProjectionSpec:
+    """Midpoint samples, sample arithmetic and resident field storage.
+
+    ``precision`` is the sampling and output arithmetic; ``storage`` is how the
+    field is held on the device.  Half storage pairs only with FP32 sampling
+    and uses compensated along-ray sums.
+    """
+
+    samples_per_ray: int
+    precision: str = "float32"
+    storage: str = "float32"
+
+    def __post_init__(self) -> None:
+        integer(self.samples_per_ray, "samples_per_ray", minimum=1)
+        if self.precision not in PRECISIONS:
+            raise ContractError("projection precision must be float32 or float64")
+        if self.storage not in STORAGES:
+            raise ContractError("storage must be float32, bfloat16 or float16")
+        if self.storage != "float32" and self.precision != "float32":
+            raise ContractError("half storage requires FP32 sampling")
+
+
+def configuration(grid: GridSpec, geometry, spec: ProjectionSpec) -> str:
+    """Encode fixed physical values as a JSON string (dispatcher-safe)."""
+    if isinstance(geometry, RayBundleGeometry):
+        views = {"rays": [list(map(list, geometry.sources_mm)), list(map(list, geometry.endpoints_mm))]}
+    else:
+        views = {"panels": [
+            {
+                "source_mm": list(p.source_mm),
+                "origin_mm": list(p.origin_mm),
+                "u": list(p.u),
+                "v": list(p.v),
+                "spacing_mm": list(p.spacing_mm),
+                "shape": list(p.shape),
+            }
+            for p in geometry.views
+        ]}
+    return json.dumps(
+        {
+            "grid": grid.encode(),
+            **views,
+            "samples": spec.samples_per_ray,
+            "precision": spec.precision,
+            "storage": spec.storage,
+        },
+        sort_keys=True,
+        separators=(",", ":"),
+        allow_nan=False,
+    )
+
+
+@dataclass(frozen=True)
+class Decoded:
+    grid: GridSpec
+    geometry: object  # BatchedGeometry | RayBundleGeometry
+    spec: ProjectionSpec
+
+    @property
+    def image_shape(self) -> tuple[int, int, int]:
+        if isinstance(self.geometry, RayBundleGeometry):
+            return (1, 1, self.geometry.rays)
+        return self.geometry.shape
+
+    @property
+    def views(self) -> int:
+        return self.image_shape[0]
+
+
+@lru_cache(maxsize=32)
+def decode(value: str) -> Decoded:
+    encoded = json.loads(value)
+    grid = GridSpec.decode(encoded["grid"])
+    if "rays" in encoded:
+        sources, endpoints = encoded["rays"]
+        geometry = RayBundleGeometry(tuple(map(tuple, sources)), tuple(map(tuple, endpoints)))
+    else:
+        geometry = BatchedGeometry(
+            tuple(
+                DetectorGeometry(
+                    tuple(p["source_mm"]),
+                    tuple(p["origin_mm"]),
+                    tuple(p["u"]),
+                    tuple(p["v"]),
+                    tuple(p["spacing_mm"]),
+                    tuple(p["shape"]),
+                )
+                for p in encoded["panels"]
+            )
+        )
+    spec = ProjectionSpec(int(encoded["samples"]), encoded["precision"], encoded.get("storage", "float32"))
+    return Decoded(grid, geometry, spec)
+
+
+@dataclass
+class Prepared:
+    plan: _native.Plan
+    decoded: Decoded
+    lock: threading.RLock = field(default_factory=threading.RLock)
+    half: object = None  # cached half-storage device buffer (torch tensor)
+
+
+_CACHE: OrderedDict[tuple, Prepared] = OrderedDict()
+_CACHE_LOCK = threading.RLock()
+
+
+def _native_geometry(decoded: Decoded):
+    geometry = decoded.geometry
+    if isinstance(geometry, RayBundleGeometry):
+        src = np.asarray(geometry.sources_mm, dtype=np.float64)
+        end = np.asarray(geometry.endpoints_mm, dtype=np.float64)
+        return dict(views=1, rows=1, cols=geometry.rays, panels=None, rays=(src, end))
+    panels = np.asarray([p.packed() for p in geometry.views], dtype=np.float64)
+    rows, cols = geometry.shape_rows_columns
+    return dict(views=len(geometry.views), rows=rows, cols=cols, panels=panels, rays=None)
+
+
+def _check_extent(decoded: Decoded) -> None:
+    """Detector corners must give finite rays (projection.py:578-586)."""
+    geometry = decoded.geometry
+    if isinstance(geometry, RayBundleGeometry):
+        return
+    for panel in geometry.views:
+        for row in (0, panel.shape[0] - 1):
+            for col in (0, panel.shape[1] - 1):
+                end = panel.pixel_centre(row, col)
+                if not all(math.isfinite(x) for x in end) or not math.isfinite(math.dist(end, panel.source_mm)):
+                    raise ContractError("detector extent exceeds the FP64 finite-ray range")
+
+
+def prepare(value: str, device: int, stream: int, needs: int) -> Prepared:
+    """Return the cached plan for this configuration, device, stream and flags."""
+    key = (value, int(device), int(stream), int(needs))
+    with _CACHE_LOCK:
+        found = _CACHE.get(key)
+        if found is not None:
+            _CACHE.move_to_end(key)
+            return found
+        decoded = decode(value)
+        _check_extent(decoded)
+        plan = _native.Plan(
+            decoded.grid,
+            samples=decoded.spec.samples_per_ray,
+            arithmetic=decoded.spec.precision,
+            storage=decoded.spec.storage,
+            needs=needs,
+            device=int(device),
+            **_native_geometry(decoded),
+        )
+        entry = Prepared(plan, decoded)
+        _CACHE[key] = entry
+        while len(_CACHE) > CACHE_LIMIT:
+            _CACHE.popitem(last=False)
+        return entry
+
+
+def clear_cache() -> None:
+    with _CACHE_LOCK:
+        _CACHE.clear()
+
+
+def consume(entry: Prepared, stream: int) -> None:
+    """One readback of every view's status word; clear, then raise the first error."""
+    codes = entry.plan.status(stream, clear=True)
+    raise_view_statuses(codes, "projection")

@@ -1,0 +1,181 @@
+"""Checked multiview Torch operators over the native HIP plan.
+
+Same operator contract as the reference (python/dpt/torch/_projection_ops.py:
+256-411), registered in the ``dpt_amd`` namespace:
+
+* ``dpt_amd::project(field, poses, value, active_volume, active_pose) -> image``
+* ``dpt_amd::project_vjp(field, poses, cotangent, value, active_volume,
+  active_pose) -> (grad_field | empty(0), grad_poses | empty(0))``
+* ``dpt_amd::project_out(field, poses, value, out!)``
+
+``value`` is the JSON configuration.  Fakes make the ops traceable; autograd
+is first order only.  Work is enqueued on the caller's current stream (one
+native plan per stream), followed by exactly one status readback per call.
+Device tensors only: there is no CPU execution path.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .. import _native
+from ..contracts import ContractError
+from ..projection import ProjectionSpec, configuration as _encode, consume, decode, prepare
+
+_TAGS = (torch.Tag.cudagraph_unsafe,)
+
+
+def configuration(grid, geometry, spec: ProjectionSpec) -> str:
+    return _encode(grid, geometry, spec)
+
+
+def _metadata(field: torch.Tensor, poses: torch.Tensor, value: str):
+    decoded = decode(value)
+    if tuple(field.shape) != decoded.grid.shape or field.dtype != torch.float32 or not field.is_contiguous():
+        raise ContractError("field must use contiguous float32 storage with grid shape")
+    if tuple(poses.shape) != (decoded.views, 12) or poses.dtype != torch.float64 or poses.device != field.device:
+        raise ContractError("poses must be float64 (views,12) on the field device")
+    dtype = torch.float64 if decoded.spec.precision == "float64" else torch.float32
+    return decoded.image_shape, dtype
+
+
+def _require_device(field: torch.Tensor) -> None:
+    if field.device.type != "cuda":
+        raise ContractError(
+            "dpt_amd executes on ROCm GPUs only; move field and poses to a cuda device"
+        )
+
+
+def _needs(active_volume: bool, active_pose: bool) -> int:
+    return (_native.VOLUME if active_volume else 0) | (_native.POSE if active_pose else 0)
+
+
+def _field_pointer(entry, field: torch.Tensor, stream: int) -> int:
+    """Pointer to the field in the plan's storage dtype (converting if half)."""
+    storage = entry.decoded.spec.storage
+    if storage == "float32":
+        return field.data_ptr()
+    dtype = torch.bfloat16 if storage == "bfloat16" else torch.float16
+    half = torch.empty(field.numel(), dtype=dtype, device=field.device)
+    _native.convert_field(field.data_ptr(), half.data_ptr(), field.numel(), storage, None, stream)
+    entry.half = half  # keep alive until the next call on this plan
+    return half.data_ptr()
+
+
+def _forward(field, poses, value, active_volume, active_pose, out=None):
+    shape, dtype = _metadata(field, poses, value)
+    if out is not None and (
+        tuple(out.shape) != shape or out.dtype != dtype or out.device != field.device or not out.is_contiguous()
+    ):
+        raise ContractError("out must match the projector's shape, dtype and input device")
+    if out is not None and any(
+        out.untyped_storage().data_ptr() == x.untyped_storage().data_ptr() for x in (field, poses)
+    ):
+        raise ContractError("out must not share storage with projection inputs")
+    _require_device(field)
+    stream = torch.cuda.current_stream(field.device).cuda_stream
+    entry = prepare(value, field.device.index or 0, stream, _needs(active_volume, active_pose))
+    with entry.lock, torch.no_grad():
+        packed = poses.contiguous()
+        images = torch.empty(shape, dtype=dtype, device=field.device) if out is None else out
+        pointer = _field_pointer(entry, field, stream)
+        entry.plan.forward(pointer, packed.data_ptr(), images.data_ptr(), stream)
+        consume(entry, stream)
+    return images
+
+
+@torch.library.custom_op("dpt_amd::project", mutates_args=(), tags=_TAGS)
+def project(
+    field: torch.Tensor, poses: torch.Tensor, value: str, active_volume: bool, active_pose: bool
+) -> torch.Tensor:
+    return _forward(field, poses, value, active_volume, active_pose)
+
+
+@project.register_fake
+def _project_fake(field, poses, value, active_volume, active_pose):
+    shape, dtype = _metadata(field, poses, value)
+    return torch.empty(shape, dtype=dtype, device=field.device)
+
+
+@torch.library.custom_op("dpt_amd::project_vjp", mutates_args=(), tags=_TAGS)
+def project_vjp(
+    field: torch.Tensor,
+    poses: torch.Tensor,
+    cotangent: torch.Tensor,
+    value: str,
+    active_volume: bool,
+    active_pose: bool,
+) -> tuple[torch.Tensor, torch.Tensor]:
+    shape, dtype = _metadata(field, poses, value)
+    if tuple(cotangent.shape) != shape or cotangent.dtype != dtype or cotangent.device != field.device:
+        raise ContractError("cotangent must match the projected image shape, dtype and device")
+    if not active_volume and not active_pose:
+        raise ContractError("request at least one projection gradient")
+    _require_device(field)
+    stream = torch.cuda.current_stream(field.device).cuda_stream
+    entry = prepare(value, field.device.index or 0, stream, _needs(active_volume, active_pose))
+    with entry.lock, torch.no_grad():
+        packed = poses.contiguous()
+        seed = cotangent.contiguous()
+        gradient = torch.empty_like(field) if active_volume else field.new_empty((0,))
+        pose_gradient = torch.empty_like(packed) if active_pose else poses.new_empty((0,))
+        flags = _needs(active_volume, active_pose)
+        pointer = _field_pointer(entry, field, stream)
+        entry.plan.vjp(
+            pointer,
+            packed.data_ptr(),
+            seed.data_ptr(),
+            gradient.data_ptr() if active_volume else None,
+            pose_gradient.data_ptr() if active_pose else None,
+            flags,
+            stream,
+        )
+        consume(entry, stream)
+    return gradient, pose_gradient
+
+
+@project_vjp.register_fake
+def _vjp_fake(field, poses, cotangent, value, active_volume, active_pose):
+    shape, dtype = _metadata(field, poses, value)
+    if tuple(cotangent.shape) != shape or cotangent.dtype != dtype or cotangent.device != field.device:
+        raise ContractError("cotangent must match the projected image shape, dtype and device")
+    return (
+        field.new_empty(field.shape if active_volume else (0,)),
+        poses.new_empty(poses.shape if active_pose else (0,)),
+    )
+
+
+def _setup_context(ctx, inputs, output):
+    field, poses, value, _, _ = inputs
+    ctx.save_for_backward(field, poses)
+    ctx.configuration = value
+
+
+def _backward(ctx, cotangent):
+    if torch.is_grad_enabled():
+        raise ContractError("the Torch projection adapter supports first-order derivatives only")
+    field, poses = ctx.saved_tensors
+    active_volume, active_pose = ctx.needs_input_grad[:2]
+    gradient, pose_gradient = project_vjp(field, poses, cotangent, ctx.configuration, active_volume, active_pose)
+    return (
+        gradient if active_volume else None,
+        pose_gradient if active_pose else None,
+        None,
+        None,
+        None,
+    )
+
+
+project.register_autograd(_backward, setup_context=_setup_context)
+
+
+@torch.library.custom_op("dpt_amd::project_out", mutates_args={"out"}, tags=_TAGS)
+def project_out(field: torch.Tensor, poses: torch.Tensor, value: str, out: torch.Tensor) -> None:
+    _forward(field, poses, value, False, False, out)
+
+
+@project_out.register_fake
+def _out_fake(field, poses, value, out):
+    shape, dtype = _metadata(field, poses, value)
+    if tuple(out.shape) != shape or out.dtype != dtype or out.device != field.device or not out.is_contiguous():
+        raise ContractError("out must match the projector's shape, dtype and input device")

@@ -1,0 +1,152 @@
+/*
+ * dpt_amd -- C ABI of the MI355X cone-beam projector.
+ *
+ * Plain pointers and sizes only: device pointers come from the caller's
+ * allocator (PyTorch, hipMalloc, ...), streams are hipStream_t passed as
+ * void*.  Every entry point returns 0 on success or a negative DPT_E_* code;
+ * dpt_last_error() then names the failure (thread-local).
+ *
+ * The reference binds this path through Warp launches behind the Torch custom
+ * operators dpt::project / dpt::project_vjp / dpt::project_out
+ * (python/dpt/torch/_projection_ops.py:256-411).  Each entry point below
+ * replaces one piece of that stack; the citation is given per declaration.
+ *
+ * Layouts (identical to the reference):
+ *   field      (nz, ny, nx) contiguous, x fastest, mm^-1
+ *              (python/dpt/volumes.py:17-25)
+ *   poses      (views, 12) float64: row-major object-to-world rotation, then
+ *              translation in mm (python/dpt/geometry.py:96-98)
+ *   images     (views, rows, cols) optical depth, row-major
+ *   grad_pose  (views, 12) float64 packed-pose cotangent, or (views, 6)
+ *              local-right tangent (tx,ty,tz,rx,ry,rz) with DPT_POSE_TANGENT
+ */
+#ifndef DPT_AMD_H
+#define DPT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPT_ABI_VERSION 1
+
+/* return codes */
+#define DPT_OK 0
+#define DPT_E_INVALID -1  /* an argument violates the contract */
+#define DPT_E_DEVICE -2   /* a HIP runtime call failed */
+#define DPT_E_MEMORY -3   /* device or pinned allocation failed */
+
+/* sample arithmetic (python/dpt/torch/modules.py:123-134) */
+#define DPT_ARITH_FP32 0  /* kernels/projection_fp32.py */
+#define DPT_ARITH_FP64 1  /* kernels/projection.py, sampling_precision=float64 */
+
+/* resident field storage; FP32 sampling reads every kind */
+#define DPT_STORE_F32 0
+#define DPT_STORE_BF16 1
+#define DPT_STORE_F16 2
+
+/* dpt_vjp flags */
+#define DPT_VOLUME 1u        /* produce grad_field */
+#define DPT_POSE 2u          /* produce grad_pose */
+#define DPT_POSE_TANGENT 4u  /* 6-component local-right tangent instead of 12 */
+#define DPT_ACCUMULATE 8u    /* add into grad_pose / grad_field instead of writing */
+#define DPT_NO_CHECKS 16u    /* skip input validation (caller validated once) */
+
+/* status bits (python/dpt/contracts.py:229-234, projection.py:505-515) */
+#define DPT_STATUS_POSE 1    /* invalid rigid pose -> ContractError */
+#define DPT_STATUS_RANGE 2   /* non-finite output/gradient -> NumericalError */
+#define DPT_STATUS_FAILURE 4 /* numerical execution failure -> NumericalError */
+#define DPT_STATUS_DOMAIN 8  /* field not finite/non-negative -> TrialDomainError */
+
+/* Grid: kernels/projection.py:51-65 Configuration.{shape, object_to_grid,
+ * grid_origin}; object_to_grid = diag(1/spacing) * orientation^T
+ * (python/dpt/projection.py:571-576). */
+typedef struct {
+    int32_t nx, ny, nz;
+    double object_to_grid[9];
+    double grid_origin[3];
+} dpt_grid_desc;
+
+typedef struct dpt_plan dpt_plan;
+
+/*
+ * Prepare a multiview plan: geometry metadata on the device, the status words,
+ * pose-reduction partials and the volume accumulator.  Replaces
+ * prepare_projection (python/dpt/projection.py:526-624) called once per view
+ * by _projection_ops.prepare (python/dpt/torch/_projection_ops.py:127-175).
+ *
+ * panels: host float64 (views, 12) = source, detector origin (pixel (0,0)
+ *   centre), column step (u * column pitch), row step (v * row pitch), or NULL.
+ * ray_sources / ray_endpoints: host float64 (views, rows*cols, 3) explicit
+ *   world segments (RayBundleGeometry, python/dpt/geometry.py:225-300), used
+ *   when panels is NULL.
+ * needs: DPT_VOLUME | DPT_POSE, the gradients later calls may request.
+ */
+int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols,
+                    const double *panels, const double *ray_sources, const double *ray_endpoints,
+                    int32_t samples, int32_t arithmetic, int32_t storage, uint32_t needs,
+                    int32_t device, dpt_plan **out);
+
+/* Release every device and pinned buffer of the plan. */
+int dpt_plan_destroy(dpt_plan *plan);
+
+/*
+ * Optical depth of every view: out (views, rows, cols), float32 for
+ * DPT_ARITH_FP32 and float64 for DPT_ARITH_FP64.  Enqueues the field and pose
+ * checks and one multiview ray-march launch on `stream`; no host sync.
+ * Replaces the per-view loop of _forward (python/dpt/torch/_projection_ops.py:
+ * 213-253) around project_optical_depth (python/dpt/projection.py:747-809)
+ * and the kernels of kernels/projection_fp32.py:334-374 and
+ * kernels/projection.py:802-846.
+ * field: float32 (nz,ny,nx) for DPT_STORE_F32, else bf16/f16 storage.
+ */
+int dpt_forward(dpt_plan *plan, const void *field, const double *poses, void *out, void *stream);
+
+/*
+ * Vector-Jacobian product for a cotangent (views, rows, cols) of the output
+ * dtype.  grad_field: float32 (nz,ny,nx) (DPT_VOLUME); grad_pose: float64
+ * (views, 12) or (views, 6) with DPT_POSE_TANGENT (DPT_POSE).
+ * Replaces project_vjp (python/dpt/torch/_projection_ops.py:273-353) around
+ * projection_vjp (python/dpt/projection.py:871-1018), the VJP kernels
+ * kernels/projection_fp32.py:377-617 / kernels/projection.py:873-1237 and the
+ * reduction tree (python/dpt/_reductions.py:27-86).
+ */
+int dpt_vjp(dpt_plan *plan, const void *field, const double *poses, const void *cotangent,
+            float *grad_field, double *grad_pose, uint32_t flags, void *stream);
+
+/*
+ * Copy the per-view status words to host_codes (views int32) with one D2H
+ * transfer and one stream synchronisation; clear them when `clear` != 0.
+ * Replaces StatusCheckpoint.check/consume (python/dpt/_runtime.py:317-445).
+ */
+int dpt_status(dpt_plan *plan, int32_t *host_codes, int32_t clear, void *stream);
+
+/* Convert float32 values to bf16 (storage DPT_STORE_BF16) or f16 with
+ * round-to-nearest-even, flagging non-finite or negative inputs with
+ * DPT_STATUS_DOMAIN in status[0] when status is not NULL.  No reference
+ * counterpart: the reference has no FP32-sampling half-storage form
+ * (python/dpt/projection.py:131-136 forbids it). */
+int dpt_convert_field(const float *src, void *dst, int64_t count, int32_t storage, int32_t *status,
+                      void *stream);
+
+/* Kernel time of the most recent launch of `kind` (0 forward, 1 VJP) on the
+ * plan, in milliseconds, measured with HIP events on the launch stream;
+ * negative when no timed launch has completed.  Enable with dpt_set_timing. */
+int dpt_set_timing(dpt_plan *plan, int32_t enabled);
+double dpt_kernel_ms(dpt_plan *plan, int32_t kind);
+
+/* Number of rays whose FP32 clipped interval is non-empty for these poses
+ * (effective samples = hits * samples).  Synchronises the stream; returns a
+ * negative DPT_E_* code on failure.  Measurement helper, no reference
+ * counterpart. */
+int64_t dpt_count_hit_rays(dpt_plan *plan, const double *poses, void *stream);
+
+const char *dpt_last_error(void);
+int32_t dpt_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPT_AMD_H */

@@ -1,0 +1,114 @@
+"""Test scenes as plain numbers, shared by oracle and product tests.
+
+Each scene restates a fixture of the reference's own tests (cited) so the
+oracle and the HIP path are compared on the same inputs.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+
+IDENTITY12 = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+
+
+def compose(anchor12, xi):
+    """anchor @ exp(xi^) with closed-form SE(3) exponential (FP64, host)."""
+    rho = np.asarray(xi[:3], dtype=np.float64)
+    w = np.asarray(xi[3:], dtype=np.float64)
+    theta2 = float(w @ w)
+    hat = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]], dtype=np.float64)
+    if theta2 < 1e-2:
+        a = 1 - theta2 / 6 + theta2**2 / 120 - theta2**3 / 5040
+        b = 0.5 - theta2 / 24 + theta2**2 / 720 - theta2**3 / 40320
+        c = 1 / 6 - theta2 / 120 + theta2**2 / 5040 - theta2**3 / 362880
+    else:
+        t = math.sqrt(theta2)
+        a, b, c = math.sin(t) / t, (1 - math.cos(t)) / theta2, (t - math.sin(t)) / (theta2 * t)
+    rot = np.eye(3) + a * hat + b * hat @ hat
+    vel = np.eye(3) + b * hat + c * hat @ hat
+    ra = np.asarray(anchor12[:9], dtype=np.float64).reshape(3, 3)
+    ta = np.asarray(anchor12[9:], dtype=np.float64)
+    return tuple((ra @ rot).reshape(-1)) + tuple(ra @ (vel @ rho) + ta)
+
+
+def oblique_scene():
+    """tests/python/warp_cpu/test_projection_fp32.py:24-68 (5x6x7 oriented grid)."""
+    angle = 0.19
+    grid = O.Grid(
+        (5, 6, 7),
+        (1.2, 0.8, 1.1),
+        (-3.2, -2.1, -1.7),
+        (math.cos(angle), -math.sin(angle), 0.0, math.sin(angle), math.cos(angle), 0.0, 0.0, 0.0, 1.0),
+    )
+    sources = np.array(
+        [(-12.0, 0.17, 0.37), (12.0, 0.23, 0.48), (-0.7, -12.0, 0.29), (0.31, 0.41, -12.0), (-12.0, 30.0, 30.0)]
+    )
+    endpoints = np.array(
+        [(12.0, 0.81, 0.63), (-12.0, 0.89, 0.91), (0.2, 12.0, 0.86), (0.72, 0.83, 12.0), (12.0, 30.0, 30.0)]
+    )
+    pose = compose(IDENTITY12, (0.07, -0.03, 0.02, 0.013, -0.021, 0.009))
+    values = np.asarray(
+        [
+            0.06 + 0.013 * x + 0.019 * y + 0.011 * z + 0.003 * x * y + 0.002 * y * z
+            for z in range(5)
+            for y in range(6)
+            for x in range(7)
+        ],
+        dtype=np.float32,
+    )
+    return grid, sources, endpoints, pose, values
+
+
+def workflow_scene(views: int = 2):
+    """tests/python/warp_cpu/test_projector_workflow.py:27-42 (3x3x4 grid, 2x3 panels)."""
+    grid = O.Grid((3, 3, 4), (0.7, 0.8, 0.9), (-1.0, -1.0, -1.0))
+    base = O.Panel((-8.0, 0.1, -0.2), (8.0, -0.3, -0.4), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (0.2, 0.2), (2, 3))
+    if views == 2:
+        panels = [base, O.Panel(base.source, (8.0, -0.24, -0.31), base.u, base.v, base.spacing, base.shape)]
+    else:
+        panels = [
+            O.Panel(base.source, (8.0, -0.3 + i * 0.03, -0.4), base.u, base.v, base.spacing, base.shape)
+            for i in range(views)
+        ]
+    field = np.linspace(0.1, 0.3, 36, dtype=np.float32)
+    return grid, panels, field
+
+
+def line_panel(source: float = -10.0, detector: float = 10.0):
+    """tests/python/test_projection_reference.py:16-24 (single pixel on the x axis)."""
+    return O.Panel((source, 0.0, 0.0), (detector, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (1.0, 1.0), (1, 1))
+
+
+def cone_panels(grid: O.Grid, views: int, rows: int, cols: int, sod=1000.0, sdd=1500.0, pitch=None, offset=0.0):
+    """Circular cone views about world z centred on the grid (geometry.py:617-682)."""
+    nx, ny, nz = grid.xyz
+    centre = [grid.origin[i] + ((grid.xyz[i] - 1) / 2) * grid.spacing[i] for i in range(3)]
+    if pitch is None:
+        diag = math.hypot(nx * grid.spacing[0], ny * grid.spacing[1], nz * grid.spacing[2])
+        pitch = 1.1 * diag * sdd / (sod - diag / 2) / max(rows, cols)
+    panels = []
+    for k in range(views):
+        a = offset + 2 * math.pi * k / views
+        c, s = math.cos(a), math.sin(a)
+
+        def rot(p):
+            return (c * p[0] - s * p[1] + centre[0], s * p[0] + c * p[1] + centre[1], p[2] + centre[2])
+
+        origin = (-0.5 * (cols - 1) * pitch, sdd - sod, -0.5 * (rows - 1) * pitch)
+        panels.append(
+            O.Panel(rot((0.0, -sod, 0.0)), rot(origin), (c, s, 0.0), (0.0, 0.0, 1.0), (pitch, pitch), (rows, cols))
+        )
+    return panels
+
+
+def smooth_field(grid: O.Grid, seed: int = 0) -> np.ndarray:
+    """Positive smooth field plus a seeded perturbation (float32)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = grid.xyz
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = 0.02 + 0.01 * np.sin(0.3 * x + 0.1) * np.cos(0.2 * y) + 0.005 * np.cos(0.25 * z)
+    return (base + 0.002 * rng.random(base.shape)).astype(np.float32)

@@ -1,0 +1,242 @@
+"""HIP projector vs the CPU oracle on identical seeded inputs (GPU).
+
+Bars: FP32 images, volume and pose gradients within 1e-5 relative L2 of the
+oracle's restatement of the reference FP32 path (per-ray quantities are
+compared much tighter); FP64 within 1e-10.  Every call goes through the C ABI
+(``dpt_amd._native.Plan``) or the ``Projector`` built on it.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import scenes
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from dpt_amd import _native  # noqa: E402
+from dpt_amd.volumes import GridSpec  # noqa: E402
+
+
+def _grid_spec(grid: O.Grid) -> GridSpec:
+    return GridSpec(tuple(grid.shape), tuple(grid.spacing), tuple(grid.origin), tuple(grid.orientation))
+
+
+class NativeScene:
+    """Drive the C ABI directly with torch-owned device buffers."""
+
+    def __init__(self, grid: O.Grid, samples: int, panels=None, rays=None, precision="float32", storage="float32"):
+        self.grid = grid
+        if panels is not None:
+            packed = np.asarray([p.packed() for p in panels], dtype=np.float64)
+            views, (rows, cols) = len(panels), panels[0].shape
+            geo = dict(panels=packed, rays=None)
+        else:
+            src, end = rays
+            views, rows, cols = 1, 1, len(src)
+            geo = dict(panels=None, rays=(np.asarray(src, np.float64), np.asarray(end, np.float64)))
+        self.shape = (views, rows, cols)
+        self.precision = precision
+        self.storage = storage
+        self.plan = _native.Plan(
+            _grid_spec(grid), views=views, rows=rows, cols=cols, samples=samples, arithmetic=precision,
+            storage=storage, needs=_native.VOLUME | _native.POSE, device=0, **geo,
+        )
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def _field(self, values):
+        f = torch.as_tensor(np.ascontiguousarray(values, dtype=np.float32)).cuda()
+        if self.storage == "float32":
+            return f, f
+        dtype = torch.bfloat16 if self.storage == "bfloat16" else torch.float16
+        half = torch.empty(f.numel(), dtype=dtype, device="cuda")
+        _native.convert_field(f.data_ptr(), half.data_ptr(), f.numel(), self.storage, None, self.stream)
+        return half, f
+
+    def _poses(self, pose):
+        pose = scenes.IDENTITY12 if pose is None else pose
+        p = np.broadcast_to(np.asarray(pose, dtype=np.float64), (self.shape[0], 12))
+        return torch.as_tensor(np.ascontiguousarray(p)).cuda()
+
+    def forward(self, values, pose=None):
+        field, _ = self._field(values)
+        poses = self._poses(pose)
+        dtype = torch.float32 if self.precision == "float32" else torch.float64
+        out = torch.empty(self.shape, dtype=dtype, device="cuda")
+        self.plan.forward(field.data_ptr(), poses.data_ptr(), out.data_ptr(), self.stream)
+        status = self.plan.status(self.stream)
+        return out.cpu().numpy(), status
+
+    def vjp(self, values, seeds, pose=None, *, volume=True, pose_grad=True, components=12):
+        field, _ = self._field(values)
+        poses = self._poses(pose)
+        dtype = torch.float32 if self.precision == "float32" else torch.float64
+        seed = torch.as_tensor(np.ascontiguousarray(seeds).reshape(self.shape)).to(dtype).cuda()
+        gv = torch.empty(self.grid.voxels, dtype=torch.float32, device="cuda")
+        gp = torch.empty((self.shape[0], components), dtype=torch.float64, device="cuda")
+        flags = (_native.VOLUME if volume else 0) | (_native.POSE if pose_grad else 0)
+        if components == 6:
+            flags |= _native.POSE_TANGENT
+        self.plan.vjp(field.data_ptr(), poses.data_ptr(), seed.data_ptr(), gv.data_ptr() if volume else None,
+                      gp.data_ptr() if pose_grad else None, flags, self.stream)
+        status = self.plan.status(self.stream)
+        return (gv.cpu().numpy() if volume else None), (gp.cpu().numpy() if pose_grad else None), status
+
+
+def _rel(a, b):
+    return O.relative_l2(a, b)
+
+
+def _cone(n=24, views=5, rows=20, cols=28, seed=3):
+    grid = O.Grid((n, n + 2, n + 4), (1.0, 0.9, 1.1), (-11.0, -12.0, -13.0))
+    panels = scenes.cone_panels(grid, views, rows, cols, sod=120.0, sdd=200.0, offset=0.3)
+    return grid, panels, scenes.smooth_field(grid, seed)
+
+
+SCENES = {
+    "oblique": lambda: scenes.oblique_scene(),
+    "workflow": lambda: scenes.workflow_scene(),
+    "cone": lambda: _cone(),
+}
+
+
+def _scene(name, samples, precision="float32"):
+    if name == "oblique":
+        grid, src, end, pose, values = scenes.oblique_scene()
+        return grid, values, pose, O.Scene(grid, samples, rays=(src, end)), NativeScene(grid, samples, rays=(src, end), precision=precision)
+    if name == "workflow":
+        grid, panels, values = scenes.workflow_scene()
+        pose = scenes.compose(scenes.IDENTITY12, (0.03, -0.02, 0.01, 0.04, -0.03, 0.02))
+    else:
+        grid, panels, values = _cone()
+        pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    return grid, values, pose, O.Scene(grid, samples, panels=panels), NativeScene(grid, samples, panels=panels, precision=precision)
+
+
+@pytest.mark.parametrize("samples", [17, 97, 256])
+@pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
+def test_fp32_forward_matches_oracle(name, samples):
+    print(f"case fp32 forward {name} S={samples}", flush=True)
+    grid, values, pose, ref, gpu = _scene(name, samples)
+    expected, s_ref = ref.forward(values, pose)
+    actual, s_gpu = gpu.forward(values, pose)
+    assert s_gpu.tolist() == s_ref.tolist() == [0] * len(s_ref)
+    assert _rel(actual, expected) <= 1e-6
+    np.testing.assert_allclose(actual, expected, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("samples", [17, 97, 256])
+@pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
+def test_fp64_forward_matches_oracle(name, samples):
+    print(f"case fp64 forward {name} S={samples}", flush=True)
+    grid, values, pose, ref, gpu = _scene(name, samples, "float64")
+    expected, _ = ref.forward(values, pose, precision="fp64")
+    actual, status = gpu.forward(values, pose)
+    assert status.tolist() == [0] * len(status)
+    np.testing.assert_allclose(actual, expected, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("components", [12, 6])
+@pytest.mark.parametrize("mode", ["both", "volume", "pose"])
+@pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
+def test_fp32_vjp_matches_oracle(name, mode, components):
+    print(f"case fp32 vjp {name} {mode} C={components}", flush=True)
+    grid, values, pose, ref, gpu = _scene(name, 97)
+    image, _ = ref.forward(values, pose)
+    rng = np.random.default_rng(7)
+    seeds = rng.uniform(-1.0, 1.0, image.size).astype(np.float32)
+    vol, pos = mode in ("both", "volume"), mode in ("both", "pose")
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose, volume=vol, pose=pos, components=components)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose, volume=vol, pose_grad=pos, components=components)
+    assert status.tolist() == [0] * len(status)
+    if vol:
+        assert _rel(g_gpu, g_ref) <= 1e-5
+    if pos:
+        assert _rel(p_gpu, p_ref) <= 1e-5
+        np.testing.assert_allclose(p_gpu, p_ref, rtol=1e-4, atol=1e-6 * np.abs(p_ref).max())
+
+
+@pytest.mark.parametrize("mode", ["both", "volume", "pose"])
+@pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
+def test_fp64_vjp_matches_oracle(name, mode):
+    print(f"case fp64 vjp {name} {mode}", flush=True)
+    grid, values, pose, ref, gpu = _scene(name, 97, "float64")
+    image, _ = ref.forward(values, pose, precision="fp64")
+    seeds = np.random.default_rng(11).uniform(-1.0, 1.0, image.size)
+    vol, pos = mode in ("both", "volume"), mode in ("both", "pose")
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose, precision="fp64", volume=vol, pose=pos)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose, volume=vol, pose_grad=pos)
+    assert status.tolist() == [0] * len(status)
+    if vol:
+        # FP32-rounded increments: GPU sums them in FP64 then rounds once to FP32
+        assert _rel(g_gpu, g_ref) <= 1e-7
+    if pos:
+        assert _rel(p_gpu, p_ref) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_closed_form_unit_voxel_through_the_c_abi(precision):
+    """tests/python/warp_cpu/test_projection_fp32.py:248-257."""
+    grid = O.Grid((1, 1, 1), (1.0, 1.0, 1.0))
+    rays = (np.array([[-2.0, 0.0, 0.0]]), np.array([[0.2, 0.0, 0.0]]))
+    gpu = NativeScene(grid, 1024, rays=rays, precision=precision)
+    image, status = gpu.forward([0.025])
+    assert status.tolist() == [0]
+    np.testing.assert_allclose(image.ravel(), [0.0175], rtol=2e-5)
+    volume, tangent, status = gpu.vjp([0.025], [1.0], components=6)
+    np.testing.assert_allclose(volume, [0.7], rtol=2e-5)
+    np.testing.assert_allclose(tangent.ravel(), [-0.025, 0, 0, 0, 0, 0], rtol=2e-5, atol=1e-8)
+
+
+def test_config1_forward_and_volume_vjp_match_oracle():
+    """64^3 phantom, AP + lateral, 128^2 at 0.8 mm, 256 samples, central-64 mean loss."""
+    from dpt_amd import phantoms
+
+    grid_spec = phantoms.centred_grid((64, 64, 64))
+    values = phantoms.chest_phantom(grid_spec, seed=1)
+    grid = O.Grid(grid_spec.shape, grid_spec.spacing_mm, grid_spec.origin_mm)
+    angles = phantoms.standard_angles()
+    panels = []
+    for name in ("ap", "lateral"):
+        a = angles[name]
+        c, s = math.cos(a), math.sin(a)
+        rows = cols = 128
+        pitch = 0.8
+        origin = (-0.5 * (cols - 1) * pitch, 500.0, -0.5 * (rows - 1) * pitch)
+        rot = lambda p: (c * p[0] - s * p[1], s * p[0] + c * p[1], p[2])  # noqa: E731
+        panels.append(O.Panel(rot((0.0, -1000.0, 0.0)), rot(origin), (c, s, 0.0), (0.0, 0.0, 1.0), (pitch, pitch), (rows, cols)))
+    ref = O.Scene(grid, 256, panels=panels)
+    gpu = NativeScene(grid, 256, panels=panels)
+    expected, _ = ref.forward(values)
+    actual, status = gpu.forward(values)
+    assert status.tolist() == [0, 0]
+    assert _rel(actual, expected) <= 1e-6
+    seeds = np.zeros((2, 128, 128), dtype=np.float32)
+    seeds[:, 32:96, 32:96] = 1.0 / (2 * 64 * 64)
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds)
+    assert status.tolist() == [0, 0]
+    assert _rel(g_gpu, g_ref) <= 1e-5
+    assert _rel(p_gpu, p_ref) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_half_storage_forward_and_gradients_within_budget_of_fp64(storage):
+    grid, panels, values = _cone(n=32, views=6, rows=24, cols=32)
+    ref = O.Scene(grid, 256, panels=panels)
+    gpu = NativeScene(grid, 256, panels=panels, storage=storage)
+    expected, _ = ref.forward(values, precision="fp64")
+    actual, status = gpu.forward(values)
+    assert status.tolist() == [0] * 6
+    assert _rel(actual, expected) <= 1e-3
+    seeds = np.random.default_rng(5).uniform(-1, 1, expected.size)
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, precision="fp64")
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds.astype(np.float32))
+    assert _rel(g_gpu, g_ref) <= 1e-3
+    assert _rel(p_gpu, p_ref) <= 1e-2
