@@ -19,8 +19,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
-import os
 import sys
 import time
 from pathlib import Path
@@ -50,7 +48,29 @@ def _args():
     parser.add_argument("--storage", default="float32", choices=["float32", "bfloat16", "float16"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-rows-stride", type=int, default=8)
+    parser.add_argument(
+        "--copy-calibration",
+        type=float,
+        default=0.0,
+        help="instead of the benchmark, run a plain device copy of this many GiB (counter calibration)",
+    )
     return parser.parse_args()
+
+
+def copy_calibration(gib: float) -> int:
+    """A plain copy kernel of known size: HBM counters of this run calibrate
+    FETCH_SIZE / WRITE_SIZE (bytes read = bytes written = the buffer size)."""
+    import torch
+
+    count = int(gib * (1 << 30)) // 4
+    src = torch.ones(count, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": count * 4, "launches": 3}), flush=True)
+    return 0
 
 
 def _log(message: str) -> None:
@@ -93,6 +113,8 @@ def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
 
 def main() -> int:
     args = _args()
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(
             json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
@@ -104,7 +126,7 @@ def main() -> int:
     import torch
 
     from dpt_amd import phantoms
-    from dpt_amd.projection import ProjectionSpec, configuration, prepare
+    from dpt_amd.projection import prepare
     from dpt_amd.torch import Projector
 
     device = torch.device("cuda:0")

@@ -28,7 +28,11 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <climits>
+#include <type_traits>
 #include <string>
+#include <vector>
 
 #include "../../include/dpt_amd.h"
 
@@ -49,8 +53,10 @@ struct Params {
     double go[3];            // grid origin
     int views, rows, cols, samples;
     int bundle;              // 0: panels, 1: explicit rays
-    int tile_w, tile_h;      // 16x16 for panels, 256x1 for bundles
-    int tiles_x, tiles_y, tiles_per_view, total_tiles, chunk;
+    int vb;                  // views per workgroup (1, 2 or 4): one wave64 per view when > 1
+    int tile_w, tile_h;      // pixels per workgroup tile of one view
+    int tiles_x, tiles_y, tiles_per_group, total_tiles, chunk;
+    int wt_x, wt_y, wt_per_view;  // wave tiles (8x8 pixels, or 64 rays) per view
     const double *panels;    // views x 12
     const double *ray_src;   // views x pixels x 3
     const double *ray_end;
@@ -72,32 +78,59 @@ __device__ __forceinline__ bool finite3d(const double *v) {
     return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
 }
 
-// Workgroup -> (view, tile). Consecutive workgroup ids land on different XCDs
-// (round robin); the remap hands each XCD a contiguous run of tiles so that
-// tiles of one view share that XCD's L2.
-__device__ __forceinline__ bool tile_of_block(const Params &P, int &view, int &tile_y, int &tile_x,
-                                              int &logical) {
+// Workgroup -> rays.  Consecutive workgroup ids land on different XCDs (round
+// robin); the remap hands each XCD a contiguous run of the tile walk (view
+// group, tile row, tile column) so the tiles an XCD has in flight sample a
+// compact part of the volume and share its L2.  A workgroup covers `vb`
+// consecutive views: vb = 1 is a 16x16 pixel tile (four 8x8 wave quadrants),
+// vb = 2 an 8x16 tile of two views, vb = 4 an 8x8 tile of four views (one wave
+// per view).  Each wave owns one 8x8 wave tile of one view; `wslot` indexes its
+// pose partials.  Explicit ray bundles use 256 consecutive rays per workgroup.
+// Returns false when the whole workgroup lies past the end of the walk.
+__device__ __forceinline__ bool lane_ray(const Params &P, int &view, int &row, int &col, int &wslot,
+                                         bool &wave_valid) {
     const int b = blockIdx.x;
-    logical = (b % kXcds) * P.chunk + b / kXcds;
+    const int logical = (b % kXcds) * P.chunk + b / kXcds;
     if (logical >= P.total_tiles) return false;
-    view = logical / P.tiles_per_view;
-    const int r = logical - view * P.tiles_per_view;
-    tile_y = r / P.tiles_x;
-    tile_x = r - tile_y * P.tiles_x;
+    const int group = logical / P.tiles_per_group;
+    const int r = logical - group * P.tiles_per_group;
+    const int ty = r / P.tiles_x;
+    const int tx = r - ty * P.tiles_x;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    if (P.bundle) {
+        view = group;
+        row = 0;
+        col = tx * kBlock + t;
+        const int wx = tx * (kBlock / 64) + w;
+        wslot = view * P.wt_per_view + wx;
+        wave_valid = wx < P.wt_x;
+        return true;
+    }
+    int vw, qy, qx;
+    if (P.vb == 1) {
+        vw = 0;
+        qy = w >> 1;
+        qx = w & 1;
+    } else if (P.vb == 2) {
+        vw = w >> 1;
+        qy = 0;
+        qx = w & 1;
+    } else {
+        vw = w;
+        qy = 0;
+        qx = 0;
+    }
+    view = group * P.vb + vw;
+    const int wy = ty * (P.tile_h >> 3) + qy, wx = tx * (P.tile_w >> 3) + qx;
+    row = wy * 8 + (l >> 3);
+    col = wx * 8 + (l & 7);
+    wslot = view * P.wt_per_view + wy * P.wt_x + wx;
+    wave_valid = view < P.views && wy < P.wt_y && wx < P.wt_x;
     return true;
 }
 
-__device__ __forceinline__ void pixel_of_lane(const Params &P, int tile_y, int tile_x, int &row,
-                                              int &col) {
-    const int t = threadIdx.x;
-    if (P.bundle) {
-        row = 0;
-        col = tile_x * kBlock + t;
-    } else {
-        const int w = t >> 6, l = t & 63;
-        row = tile_y * kTile + (w >> 1) * 8 + (l >> 3);
-        col = tile_x * kTile + (w & 1) * 8 + (l & 7);
-    }
+__device__ __forceinline__ bool ray_in_range(const Params &P, int view, int row, int col) {
+    return view < P.views && row < P.rows && col < P.cols;
 }
 
 // ------------------------------------------------------------- ray setup --
@@ -255,10 +288,27 @@ __device__ __forceinline__ void weights32(const Cell32 &c, float *w) {
     for (int k = 0; k < 8; ++k) w[k] = xy[k & 3] * ((k & 4) ? c.fz : az);
 }
 
-template <typename T>
-__device__ __forceinline__ void load8(const Params &P, const T *field, const Cell32 &c, float *v) {
+// Eight corner values.  With PAIR (FP32 storage, nx > 1) each x-adjacent pair
+// (lower x, lower x + 1 <= nx - 1) is one dword-aligned 8-byte load: four
+// loads per sample instead of eight, never past the end of an x row.
+struct __attribute__((packed, aligned(4))) FloatPair {
+    float a, b;
+};
+
+template <typename T, bool PAIR>
+__device__ __forceinline__ void load8(const Params &P, const T *field, int base, float *v) {
+    if constexpr (PAIR && std::is_same<T, float>::value) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = load_value<T>(field, c.base + corner_offset(P, k));
+        for (int j = 0; j < 4; ++j) {
+            const FloatPair q =
+                *reinterpret_cast<const FloatPair *>(field + base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0));
+            v[2 * j] = q.a;
+            v[2 * j + 1] = q.b;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = load_value<T>(field, base + corner_offset(P, k));
+    }
 }
 
 // sum_k fma(w_k, v_k, .) in corner order (kernels/projection_fp32.py:165-168)
@@ -320,14 +370,13 @@ struct Acc<true> {
 // ------------------------------------------------------- FP32 forward ------
 // kernels/projection_fp32.py:334-374
 
-template <typename T, bool COMPENSATED>
+template <typename T, bool COMPENSATED, bool PAIR>
 __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__restrict__ field,
                                                            float *__restrict__ out) {
-    int view, ty, tx, logical;
-    if (!tile_of_block(P, view, ty, tx, logical)) return;
-    int row, col;
-    pixel_of_lane(P, ty, tx, row, col);
-    if (row >= P.rows || col >= P.cols) return;
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
+    if (!ray_in_range(P, view, row, col)) return;
     const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
     const double *pose = P.poses + 12 * (size_t)view;
     World w;
@@ -361,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
             if (cell32(P, p0, p1, p2, c)) {
                 float wt[8], v[8];
                 weights32(c, wt);
-                load8<T>(P, field, c, v);
+                load8<T, PAIR>(P, field, c.base, v);
                 value = interpolate32(wt, v);
             }
             total.add(value);
@@ -381,21 +430,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum C components over the 256 lanes in a fixed order; lane c < C of the
-// block writes partial[c].
+// Sum C components over the 64 lanes of the wave (fixed shuffle tree); lane 0
+// writes the wave tile's partials.
 template <int C>
-__device__ __forceinline__ void block_store(const double *g, double *partial) {
-    __shared__ double red[kBlock / 64][kMaxComponents];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+__device__ __forceinline__ void wave_store(const double *g, double *partial, bool valid) {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double s = wave_sum(g[c]);
-        if (l == 0) red[w][c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < C) {
-        const int c = threadIdx.x;
-        partial[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+        if (valid && (threadIdx.x & 63) == 0) partial[c] = s;
     }
 }
 
@@ -432,166 +474,328 @@ __device__ __forceinline__ void pose_epilogue(const Params &P, const double *pos
 }
 
 // Register cell accumulator for the volume cotangent (reference Vec8 sums,
-// kernels/projection_fp32.py:261-269), flushed to the FP64 accumulator.
+// kernels/projection_fp32.py:261-269): increments of consecutive samples in one
+// interpolation cell are summed in FP32 registers, then flushed once.
 struct CellSums {
-    int base;
+    int x, y, z;   // lower corner; x < 0 when empty
     float s[8];
     __device__ __forceinline__ void reset() {
-        base = -1;
+        x = -1;
 #pragma unroll
         for (int k = 0; k < 8; ++k) s[k] = 0.0f;
     }
-    __device__ __forceinline__ void flush(const Params &P, double *acc) {
-        if (base < 0) return;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)s[k]);
-        reset();
-    }
 };
 
-// ---------------------------------------------------------- FP32 VJP ------
-// kernels/projection_fp32.py:377-617
+// Flush to the FP64 global accumulator.
+__device__ __forceinline__ void flush_global(const Params &P, CellSums &cs, double *acc) {
+    if (cs.x < 0) return;
+    const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (cs.s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.s[k]);
+    cs.reset();
+}
 
-template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED>
+// Brick = axis-aligned box of grid samples that one workgroup's rays touch
+// during one chunk of samples, privatised in LDS.
+struct Brick {
+    int x0, y0, z0;      // first sample of the box
+    int bx, by, bz;      // extents
+    int lsx, lsy, lsz;   // local corner strides (0 on singleton grid axes)
+    int volume;
+};
+
+__device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, float *brick) {
+    if (cs.x < 0) return;
+    const int base = ((cs.z - B.z0) * B.by + (cs.y - B.y0)) * B.bx + (cs.x - B.x0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
+        if (cs.s[k] != 0.0f) atomicAdd(&brick[base + off], cs.s[k]);
+    }
+    cs.reset();
+}
+
+constexpr int kChunk = 32;         // samples per lockstep chunk
+constexpr int kBrickCap = 8192;    // LDS floats per workgroup (32 KiB)
+
+// Clamped lower cell coordinate of one FP32 sample (same arithmetic as cell32,
+// without the support test: out-of-support samples only widen the box).
+__device__ __forceinline__ void lower_cell32(const Params &P, float p0, float p1, float p2, int *l) {
+    l[0] = min((int)floorf(fminf(fmaxf(p0, 0.0f), P.cx)), P.capx);
+    l[1] = min((int)floorf(fminf(fmaxf(p1, 0.0f), P.cy)), P.capy);
+    l[2] = min((int)floorf(fminf(fmaxf(p2, 0.0f), P.cz)), P.capz);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// ---------------------------------------------------------- FP32 VJP ------
+// kernels/projection_fp32.py:377-617.  The workgroup walks its rays' samples
+// in lockstep chunks.  Per chunk it bounds the cells its rays touch (sample
+// positions are monotone in the sample index, so the first and last sample of
+// the chunk bound each ray), privatises that brick in LDS when it fits, sums
+// register cell accumulators into it with LDS atomics, then adds the non-zero
+// brick entries to the FP64 accumulator with one global atomic each.  Chunks
+// whose brick does not fit use global atomics directly.
+
+template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, bool PAIR>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
                                                        const float *__restrict__ seeds,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ partials) {
-    int view, ty, tx, logical;
-    if (!tile_of_block(P, view, ty, tx, logical)) return;
-    int row, col;
-    pixel_of_lane(P, ty, tx, row, col);
+    __shared__ float brick[VOL ? kBrickCap : 1];
+    __shared__ int bound_red[kBlock / 64][6];
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
     double gradient[kMaxComponents];
 #pragma unroll
     for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
     const double *pose = P.poses + 12 * (size_t)view;
-    if (row < P.rows && col < P.cols) {
+
+    // ---- per-ray setup
+    bool active = false, bounded = false;
+    float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, len = 0.0f, seed = 0.0f, span = 0.0f, step = 0.0f;
+    float volume_factor = 0.0f;
+    Interval32 iv{};
+    if (ray_in_range(P, view, row, col)) {
         const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
         World w;
         ray_world(P, pose, view, row, col, w);
         Ray r;
         ray_grid(P, w, r);
-        float o[3] = {(float)r.o[0], (float)r.o[1], (float)r.o[2]};
-        float d[3] = {(float)r.d[0], (float)r.d[1], (float)r.d[2]};
-        const float len = (float)r.len;
-        if (!r.valid || !finite3f(o[0], o[1], o[2]) || !finite3f(d[0], d[1], d[2]) ||
-            !isfinite(len) || len <= 0.0f) {
+        o[0] = (float)r.o[0];
+        o[1] = (float)r.o[1];
+        o[2] = (float)r.o[2];
+        d[0] = (float)r.d[0];
+        d[1] = (float)r.d[1];
+        d[2] = (float)r.d[2];
+        len = (float)r.len;
+        if (!r.valid || !finite3f(o[0], o[1], o[2]) || !finite3f(d[0], d[1], d[2]) || !isfinite(len) ||
+            len <= 0.0f) {
             flag(P, view, DPT_STATUS_RANGE);
         } else {
-            const Interval32 iv = interval32(P, o, d);
-            const float span = iv.upper - iv.lower;
-            const float seed = seeds[ray];
+            iv = interval32(P, o, d);
+            span = iv.upper - iv.lower;
+            seed = seeds[ray];
             if (!isfinite(seed)) flag(P, view, DPT_STATUS_RANGE);
             if (span > 0.0f && seed != 0.0f) {
-                const float step = span / (float)P.samples;
-                const float physical_step = len * step;
-                const float volume_factor = seed * physical_step;
-                const bool bounded = isfinite(volume_factor) && fabsf(volume_factor) <= 1.0f;
-                Acc<COMPENSATED> mean, dor0, dor1, dor2, shl, shu;
-                float ddir[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
-                mean.init();
-                dor0.init();
-                dor1.init();
-                dor2.init();
-                shl.init();
-                shu.init();
-                CellSums cs;
-                cs.reset();
-                bool bad = false;
-                for (int k = 0; k < P.samples; ++k) {
-                    const float pos = fmaf((float)k + 0.5f, step, iv.lower);
-                    const float p0 = fmaf(pos, d[0], o[0]);
-                    const float p1 = fmaf(pos, d[1], o[1]);
-                    const float p2 = fmaf(pos, d[2], o[2]);
-                    bad |= !finite3f(p0, p1, p2);
-                    Cell32 c;
-                    const bool inside = cell32(P, p0, p1, p2, c);
-                    float wt[8];
-                    if (inside) weights32(c, wt);
-                    if constexpr (POSE) {
-                        float value = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
-                        if (inside) {
-                            float v[8];
-                            load8<T>(P, field, c, v);
-                            value = interpolate32(wt, v);
-                            slopes32(c, v, g);
-                        }
-                        mean.add(value);
-                        dor0.add(g[0]);
-                        dor1.add(g[1]);
-                        dor2.add(g[2]);
-                        if constexpr (COMPENSATED) {
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                // compensated fma accumulation of pos * g
-                                const float y = fmaf(pos, g[a], -ddc[a]);
-                                const float t = ddir[a] + y;
-                                ddc[a] = (t - ddir[a]) - y;
-                                ddir[a] = t;
-                            }
-                        } else {
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) ddir[a] = fmaf(pos, g[a], ddir[a]);
-                        }
-                        const float alpha = ((float)k + 0.5f) / (float)P.samples;
-                        const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
-                        if constexpr (COMPENSATED) {
-                            shl.add((1.0f - alpha) * along);
-                            shu.add(alpha * along);
-                        } else {
-                            shl.s = fmaf(1.0f - alpha, along, shl.s);
-                            shu.s = fmaf(alpha, along, shu.s);
-                        }
-                    }
-                    if constexpr (VOL) {
-                        if (inside) {
-                            if (bounded) {
-                                if (c.base != cs.base) {
-                                    cs.flush(P, acc);
-                                    cs.base = c.base;
-                                }
-#pragma unroll
-                                for (int kk = 0; kk < 8; ++kk)
-                                    cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
-                            } else {
-#pragma unroll
-                                for (int kk = 0; kk < 8; ++kk) {
-                                    const float inc = scatter_increment(volume_factor, wt[kk]);
-                                    if (!isfinite(inc)) bad = true;
-                                    if (inc != 0.0f)
-                                        unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
-                                }
-                            }
-                        } else {
-                            cs.flush(P, acc);
-                        }
-                    }
-                }
-                if constexpr (VOL) cs.flush(P, acc);
-                if (bad) flag(P, view, DPT_STATUS_RANGE);
-                if constexpr (POSE) {
-                    const float m = mean.value();
-                    const float lower_seed = fmaf(span, shl.value(), -m);
-                    const float upper_seed = fmaf(span, shu.value(), m);
-                    const float dor[3] = {dor0.value(), dor1.value(), dor2.value()};
-                    double og[3], dg[3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0f;
-                        const float lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0f;
-                        const float up_o = (iv.up_axis == a) ? iv.up_o : 0.0f;
-                        const float up_d = (iv.up_axis == a) ? iv.up_d : 0.0f;
-                        og[a] = (double)fmaf(upper_seed, up_o, fmaf(lower_seed, lo_o, span * dor[a]));
-                        dg[a] = (double)fmaf(upper_seed, up_d, fmaf(lower_seed, lo_d, span * ddir[a]));
-                    }
-                    const float factor = (seed * len) / (float)P.samples;
-                    pose_epilogue<C>(P, pose, w, (double)factor, og, dg, gradient);
-                }
+                active = true;
+                step = span / (float)P.samples;
+                volume_factor = seed * (len * step);
+                bounded = isfinite(volume_factor) && fabsf(volume_factor) <= 1.0f;
             }
         }
     }
-    if constexpr (POSE) block_store<C>(gradient, partials + (size_t)logical * C);
+
+    Acc<COMPENSATED> mean, dor0, dor1, dor2, shl, shu;
+    float ddir[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
+    mean.init();
+    dor0.init();
+    dor1.init();
+    dor2.init();
+    shl.init();
+    shu.init();
+    CellSums cs;
+    cs.reset();
+    bool bad = false;
+
+    if constexpr (VOL) {
+        for (int i = threadIdx.x; i < kBrickCap; i += kBlock) brick[i] = 0.0f;
+    }
+    const int chunks = (P.samples + kChunk - 1) / kChunk;
+    for (int ch = 0; ch < chunks; ++ch) {
+        const int k0 = ch * kChunk;
+        const int k1 = min(k0 + kChunk, P.samples);
+        Brick B{};
+        bool use_lds = false;
+        if constexpr (VOL) {
+            // ---- bound the cells of this chunk over the workgroup
+            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+            if (active && bounded) {
+                const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
+                const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
+                int la[3], lb[3];
+                lower_cell32(P, fmaf(pa, d[0], o[0]), fmaf(pa, d[1], o[1]), fmaf(pa, d[2], o[2]), la);
+                lower_cell32(P, fmaf(pb, d[0], o[0]), fmaf(pb, d[1], o[1]), fmaf(pb, d[2], o[2]), lb);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = min(la[a], lb[a]);
+                    nhi[a] = -max(la[a], lb[a]);
+                }
+            }
+            const int wv = threadIdx.x >> 6;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = wave_min(lo[a]);
+                nhi[a] = wave_min(nhi[a]);
+            }
+            if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    bound_red[wv][a] = lo[a];
+                    bound_red[wv][3 + a] = nhi[a];
+                }
+            }
+            __syncthreads();
+            int l[3], h[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                l[a] = min(min(bound_red[0][a], bound_red[1][a]), min(bound_red[2][a], bound_red[3][a]));
+                h[a] = -min(min(bound_red[0][3 + a], bound_red[1][3 + a]), min(bound_red[2][3 + a], bound_red[3][3 + a]));
+            }
+            if (l[0] != INT_MAX) {
+                const int n[3] = {P.nx, P.ny, P.nz};
+                int e[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
+                const long long vol = (long long)e[0] * e[1] * e[2];
+                if (vol <= kBrickCap) {
+                    use_lds = true;
+                    B.x0 = l[0];
+                    B.y0 = l[1];
+                    B.z0 = l[2];
+                    B.bx = e[0];
+                    B.by = e[1];
+                    B.bz = e[2];
+                    B.lsx = P.nx > 1 ? 1 : 0;
+                    B.lsy = P.ny > 1 ? e[0] : 0;
+                    B.lsz = P.nz > 1 ? e[0] * e[1] : 0;
+                    B.volume = (int)vol;
+                }
+            }
+        }
+        if (active) {
+            for (int k = k0; k < k1; ++k) {
+                const float pos = fmaf((float)k + 0.5f, step, iv.lower);
+                const float p0 = fmaf(pos, d[0], o[0]);
+                const float p1 = fmaf(pos, d[1], o[1]);
+                const float p2 = fmaf(pos, d[2], o[2]);
+                bad |= !finite3f(p0, p1, p2);
+                Cell32 c;
+                const bool inside = cell32(P, p0, p1, p2, c);
+                float wt[8];
+                if (inside) weights32(c, wt);
+                if constexpr (POSE) {
+                    float value = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+                    if (inside) {
+                        float v[8];
+                        load8<T, PAIR>(P, field, c.base, v);
+                        value = interpolate32(wt, v);
+                        slopes32(c, v, g);
+                    }
+                    mean.add(value);
+                    dor0.add(g[0]);
+                    dor1.add(g[1]);
+                    dor2.add(g[2]);
+                    if constexpr (COMPENSATED) {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const float y = fmaf(pos, g[a], -ddc[a]);
+                            const float t = ddir[a] + y;
+                            ddc[a] = (t - ddir[a]) - y;
+                            ddir[a] = t;
+                        }
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) ddir[a] = fmaf(pos, g[a], ddir[a]);
+                    }
+                    const float alpha = ((float)k + 0.5f) / (float)P.samples;
+                    const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
+                    if constexpr (COMPENSATED) {
+                        shl.add((1.0f - alpha) * along);
+                        shu.add(alpha * along);
+                    } else {
+                        shl.s = fmaf(1.0f - alpha, along, shl.s);
+                        shu.s = fmaf(alpha, along, shu.s);
+                    }
+                }
+                if constexpr (VOL) {
+                    if (inside) {
+                        if (bounded) {
+                            if (c.lx != cs.x || c.ly != cs.y || c.lz != cs.z) {
+                                if (use_lds)
+                                    flush_lds(B, cs, brick);
+                                else
+                                    flush_global(P, cs, acc);
+                                cs.x = c.lx;
+                                cs.y = c.ly;
+                                cs.z = c.lz;
+                            }
+#pragma unroll
+                            for (int kk = 0; kk < 8; ++kk) cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
+                        } else {
+#pragma unroll
+                            for (int kk = 0; kk < 8; ++kk) {
+                                const float inc = scatter_increment(volume_factor, wt[kk]);
+                                if (!isfinite(inc)) bad = true;
+                                if (inc != 0.0f) unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
+                            }
+                        }
+                    } else if (use_lds) {
+                        flush_lds(B, cs, brick);
+                    } else {
+                        flush_global(P, cs, acc);
+                    }
+                }
+            }
+        }
+        if constexpr (VOL) {
+            if (use_lds) {
+                flush_lds(B, cs, brick);
+                __syncthreads();
+                // drain the brick: one FP64 global atomic per touched sample
+                const float inv_bx = 1.0f / (float)B.bx, inv_by = 1.0f / (float)B.by;
+                for (int i = threadIdx.x; i < B.volume; i += kBlock) {
+                    const float v = brick[i];
+                    if (v != 0.0f) {
+                        brick[i] = 0.0f;
+                        int q = (int)((float)i * inv_bx);
+                        q -= (q * B.bx > i) ? 1 : 0;
+                        q += ((q + 1) * B.bx <= i) ? 1 : 0;
+                        const int x = i - q * B.bx;
+                        int z = (int)((float)q * inv_by);
+                        z -= (z * B.by > q) ? 1 : 0;
+                        z += ((z + 1) * B.by <= q) ? 1 : 0;
+                        const int y = q - z * B.by;
+                        const int g = ((B.z0 + z) * P.ny + (B.y0 + y)) * P.nx + (B.x0 + x);
+                        unsafeAtomicAdd(&acc[g], (double)v);
+                    }
+                }
+            } else {
+                flush_global(P, cs, acc);
+            }
+            __syncthreads();
+        }
+    }
+    if (bad) flag(P, view, DPT_STATUS_RANGE);
+    if constexpr (POSE) {
+        if (active) {
+            const float m = mean.value();
+            const float lower_seed = fmaf(span, shl.value(), -m);
+            const float upper_seed = fmaf(span, shu.value(), m);
+            const float dor[3] = {dor0.value(), dor1.value(), dor2.value()};
+            double og[3], dg[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0f;
+                const float lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0f;
+                const float up_o = (iv.up_axis == a) ? iv.up_o : 0.0f;
+                const float up_d = (iv.up_axis == a) ? iv.up_d : 0.0f;
+                og[a] = (double)fmaf(upper_seed, up_o, fmaf(lower_seed, lo_o, span * dor[a]));
+                dg[a] = (double)fmaf(upper_seed, up_d, fmaf(lower_seed, lo_d, span * ddir[a]));
+            }
+            const float factor = (seed * len) / (float)P.samples;
+            World w;  // recomputed: keeps 24 FP64 registers out of the sample loop
+            ray_world(P, pose, view, row, col, w);
+            pose_epilogue<C>(P, pose, w, (double)factor, og, dg, gradient);
+        }
+        wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+    }
 }
 
 // --------------------------------------------------------- FP64 sampling --
@@ -702,11 +906,10 @@ __device__ __forceinline__ void load8d(const Params &P, const T *field, int base
 template <typename T>
 __global__ __launch_bounds__(kBlock) void forward64_kernel(Params P, const T *__restrict__ field,
                                                            double *__restrict__ out) {
-    int view, ty, tx, logical;
-    if (!tile_of_block(P, view, ty, tx, logical)) return;
-    int row, col;
-    pixel_of_lane(P, ty, tx, row, col);
-    if (row >= P.rows || col >= P.cols) return;
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
+    if (!ray_in_range(P, view, row, col)) return;
     const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
     const double *pose = P.poses + 12 * (size_t)view;
     World w;
@@ -769,15 +972,14 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                                                        const double *__restrict__ seeds,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ partials) {
-    int view, ty, tx, logical;
-    if (!tile_of_block(P, view, ty, tx, logical)) return;
-    int row, col;
-    pixel_of_lane(P, ty, tx, row, col);
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
     double gradient[kMaxComponents];
 #pragma unroll
     for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
     const double *pose = P.poses + 12 * (size_t)view;
-    if (row < P.rows && col < P.cols) {
+    if (ray_in_range(P, view, row, col)) {
         const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
         World w;
         ray_world(P, pose, view, row, col, w);
@@ -873,7 +1075,7 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
             }
         }
     }
-    if constexpr (POSE) block_store<C>(gradient, partials + (size_t)logical * C);
+    if constexpr (POSE) wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
 }
 
 // ------------------------------------------------------- small kernels ----
@@ -993,12 +1195,11 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(const float *__restrict
 // Count rays whose FP32 clipped interval is non-empty (effective samples =
 // count * samples); same setup arithmetic as the forward kernel.
 __global__ __launch_bounds__(kBlock) void count_hits_kernel(Params P, unsigned long long *count) {
-    int view, ty, tx, logical;
-    if (!tile_of_block(P, view, ty, tx, logical)) return;
-    int row, col;
-    pixel_of_lane(P, ty, tx, row, col);
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
     bool hit = false;
-    if (row < P.rows && col < P.cols) {
+    if (ray_in_range(P, view, row, col)) {
         const double *pose = P.poses + 12 * (size_t)view;
         World w;
         ray_world(P, pose, view, row, col, w);
@@ -1033,6 +1234,7 @@ struct dpt_plan {
     double *d_partials = nullptr;
     double *d_acc = nullptr;
     unsigned long long *d_count = nullptr;
+    int vjp_vb = 1;          // views per workgroup of the VJP launches
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed[2] = {false, false};
@@ -1073,6 +1275,62 @@ unsigned grid_blocks(int64_t n) {
     return (unsigned)b;
 }
 
+// Tile geometry for `vb` views per workgroup (see lane_ray).
+void configure_tiles(Params &P, int vb) {
+    P.vb = P.bundle ? 1 : vb;
+    if (P.bundle) {
+        P.tile_w = kBlock;
+        P.tile_h = 1;
+        P.wt_x = (P.cols + 63) / 64;
+        P.wt_y = 1;
+    } else {
+        P.tile_w = P.vb == 4 ? 8 : 16;
+        P.tile_h = P.vb == 1 ? 16 : 8;
+        P.wt_x = (P.cols + 7) / 8;
+        P.wt_y = (P.rows + 7) / 8;
+    }
+    P.wt_per_view = P.wt_x * P.wt_y;
+    P.tiles_x = (P.cols + P.tile_w - 1) / P.tile_w;
+    P.tiles_y = (P.rows + P.tile_h - 1) / P.tile_h;
+    P.tiles_per_group = P.tiles_x * P.tiles_y;
+    P.total_tiles = P.tiles_per_group * ((P.views + P.vb - 1) / P.vb);
+    P.chunk = (P.total_tiles + kXcds - 1) / kXcds;
+}
+
+// Views share a workgroup (and its LDS brick) only when their central rays
+// differ by at most ~4 degrees: then their samples of one chunk overlap.
+int choose_views_per_block(const double *panels, int views, int rows, int cols) {
+    if (views < 2) return 1;
+    std::vector<double> dir(3 * (size_t)views);
+    const int n = views;
+    for (int v = 0; v < n; ++v) {
+        const double *pn = panels + 12 * (size_t)v;
+        double len = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            const double centre = pn[3 + i] + 0.5 * (cols - 1) * pn[6 + i] + 0.5 * (rows - 1) * pn[9 + i];
+            dir[3 * v + i] = centre - pn[i];
+            len += dir[3 * v + i] * dir[3 * v + i];
+        }
+        len = std::sqrt(len);
+        for (int i = 0; i < 3; ++i) dir[3 * v + i] /= len;
+    }
+    const double cos_limit = std::cos(4.0 * 3.14159265358979323846 / 180.0);
+    for (int vb : {4, 2}) {
+        bool ok = true;
+        for (int g = 0; g + 1 < n && ok; g += vb)
+            for (int a = g; a < g + vb && a < n && ok; ++a)
+                for (int b = a + 1; b < g + vb && b < n; ++b) {
+                    const double c = dir[3 * a] * dir[3 * b] + dir[3 * a + 1] * dir[3 * b + 1] + dir[3 * a + 2] * dir[3 * b + 2];
+                    if (c < cos_limit) {
+                        ok = false;
+                        break;
+                    }
+                }
+        if (ok) return vb;
+    }
+    return 1;
+}
+
 Params launch_params(const dpt_plan *p, const double *poses) {
     Params P = p->base;
     P.poses = poses;
@@ -1085,19 +1343,37 @@ unsigned tile_grid(const Params &P) { return (unsigned)(P.chunk * kXcds); }
 template <typename T>
 int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, hipStream_t s) {
     const T *f = static_cast<const T *>(field);
+    const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
-        if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL((forward32_kernel<T, false>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
-                               static_cast<float *>(out));
-        else
-            hipLaunchKernelGGL((forward32_kernel<T, true>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
-                               static_cast<float *>(out));
+        float *o = static_cast<float *>(out);
+        if constexpr (std::is_same<T, float>::value) {
+            if (P.nx > 1)
+                hipLaunchKernelGGL((forward32_kernel<T, false, true>), grid, block, 0, s, P, f, o);
+            else
+                hipLaunchKernelGGL((forward32_kernel<T, false, false>), grid, block, 0, s, P, f, o);
+        } else {
+            hipLaunchKernelGGL((forward32_kernel<T, true, false>), grid, block, 0, s, P, f, o);
+        }
     } else {
-        hipLaunchKernelGGL((forward64_kernel<T>), dim3(tile_grid(P)), dim3(kBlock), 0, s, P, f,
-                           static_cast<double *>(out));
+        hipLaunchKernelGGL((forward64_kernel<T>), grid, block, 0, s, P, f, static_cast<double *>(out));
     }
     HIP_TRY(hipGetLastError());
     return DPT_OK;
+}
+
+template <typename T, int C, bool PAIR>
+void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, hipStream_t s) {
+    constexpr bool HALF = !std::is_same<T, float>::value;
+    const dim3 grid(tile_grid(P)), block(kBlock);
+    if (vol && pose)
+        hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
+                           p->d_partials);
+    else if (vol)
+        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
+                           p->d_partials);
+    else
+        hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
+                           p->d_partials);
 }
 
 template <typename T, int C>
@@ -1105,18 +1381,16 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
                  hipStream_t s) {
     const T *f = static_cast<const T *>(field);
     const dim3 grid(tile_grid(P)), block(kBlock);
-    constexpr bool HALF = !std::is_same<T, float>::value;
     if (p->arithmetic == DPT_ARITH_FP32) {
         const float *sd = static_cast<const float *>(seeds);
-        if (vol && pose)
-            hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
-        else if (vol)
-            hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
-        else
-            hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+        if constexpr (std::is_same<T, float>::value) {
+            if (P.nx > 1)
+                launch_vjp32<T, C, true>(p, P, f, sd, vol, pose, s);
+            else
+                launch_vjp32<T, C, false>(p, P, f, sd, vol, pose, s);
+        } else {
+            launch_vjp32<T, C, false>(p, P, f, sd, vol, pose, s);
+        }
     } else {
         const double *sd = static_cast<const double *>(seeds);
         if (vol && pose)
@@ -1231,18 +1505,12 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     P.cols = cols;
     P.samples = samples;
     P.bundle = panels == nullptr ? 1 : 0;
-    if (P.bundle) {
-        P.tile_w = kBlock;
-        P.tile_h = 1;
-    } else {
-        P.tile_w = kTile;
-        P.tile_h = kTile;
+    configure_tiles(P, 1);
+    p->vjp_vb = P.bundle ? 1 : choose_views_per_block(panels, views, rows, cols);
+    if (const char *vb = std::getenv("DPT_AMD_VIEWS_PER_BLOCK")) {
+        const int forced = std::atoi(vb);
+        if (!P.bundle && (forced == 1 || forced == 2 || forced == 4)) p->vjp_vb = forced;
     }
-    P.tiles_x = (cols + P.tile_w - 1) / P.tile_w;
-    P.tiles_y = (rows + P.tile_h - 1) / P.tile_h;
-    P.tiles_per_view = P.tiles_x * P.tiles_y;
-    P.total_tiles = P.tiles_per_view * views;
-    P.chunk = (P.total_tiles + kXcds - 1) / kXcds;
 
     auto cleanup = [&](int code, const std::string &msg) {
         dpt_plan_destroy(p);
@@ -1274,7 +1542,7 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         return cleanup(DPT_E_DEVICE, "status clear failed");
     if (hipHostMalloc((void **)&p->h_status, sizeof(int32_t) * views, hipHostMallocDefault) != hipSuccess)
         return cleanup(DPT_E_MEMORY, "pinned status allocation failed");
-    if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.total_tiles);
+    if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * views);
     if (needs & DPT_VOLUME) ALLOC(p->d_acc, sizeof(double) * (size_t)voxels);
     ALLOC(p->d_count, sizeof(unsigned long long));
     for (auto &e : p->ev)
@@ -1330,7 +1598,8 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     const int components = (flags & DPT_POSE_TANGENT) ? 6 : 12;
     DeviceGuard guard(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const Params P = launch_params(p, poses);
+    Params P = launch_params(p, poses);
+    configure_tiles(P, p->vjp_vb);
     if (!(flags & DPT_NO_CHECKS)) {
         switch (p->storage) {
         case DPT_STORE_F32:
@@ -1371,10 +1640,10 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     if (pose) {
         if (components == 6)
             hipLaunchKernelGGL(pose_reduce_kernel<6>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.tiles_per_view, grad_pose, accumulate, p->d_status);
+                               P.wt_per_view, grad_pose, accumulate, p->d_status);
         else
             hipLaunchKernelGGL(pose_reduce_kernel<12>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.tiles_per_view, grad_pose, accumulate, p->d_status);
+                               P.wt_per_view, grad_pose, accumulate, p->d_status);
     }
     if (vol)
         hipLaunchKernelGGL(finish_volume_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc,

@@ -180,6 +180,31 @@ def test_fp64_vjp_matches_oracle(name, mode):
         assert _rel(p_gpu, p_ref) <= 1e-10
 
 
+@pytest.mark.parametrize("views_per_block", ["1", "2", "4"])
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_vjp_views_per_workgroup_layouts_match_oracle(views_per_block, precision, monkeypatch):
+    """Adjacent views (1 degree apart) sharing one workgroup and LDS brick; an
+    odd view count leaves a partial last group; rows/cols not multiples of 8."""
+    print(f"case vb={views_per_block} {precision}", flush=True)
+    monkeypatch.setenv("DPT_AMD_VIEWS_PER_BLOCK", views_per_block)
+    grid = O.Grid((20, 22, 24), (1.0, 1.0, 1.0), (-11.5, -10.5, -9.5))
+    panels = [scenes.cone_panels(grid, 360, 21, 27, sod=120.0, sdd=200.0, offset=0.1)[k] for k in range(9)]
+    values = scenes.smooth_field(grid, 9)
+    ref = O.Scene(grid, 97, panels=panels)
+    gpu = NativeScene(grid, 97, panels=panels, precision=precision)
+    fp = "fp32" if precision == "float32" else "fp64"
+    image, _ = ref.forward(values, precision=fp)
+    seeds = np.random.default_rng(2).uniform(-1, 1, image.size)
+    if precision == "float32":
+        seeds = seeds.astype(np.float32)
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, precision=fp)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds)
+    assert status.tolist() == [0] * 9
+    bar = 1e-5 if precision == "float32" else 1e-7
+    assert _rel(g_gpu, g_ref) <= bar
+    assert _rel(p_gpu, p_ref) <= (1e-5 if precision == "float32" else 1e-10)
+
+
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 def test_closed_form_unit_voxel_through_the_c_abi(precision):
     """tests/python/warp_cpu/test_projection_fp32.py:248-257."""
