@@ -47,7 +47,12 @@ def _args():
     parser.add_argument("--samples", type=int, default=SAMPLES)
     parser.add_argument("--storage", default="float32", choices=["float32", "bfloat16", "float16"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
-    parser.add_argument("--cpu-rows-stride", type=int, default=8)
+    parser.add_argument("--cpu-rows-stride", type=int, default=1)
+    parser.add_argument(
+        "--kernels-once",
+        action="store_true",
+        help="instead of the benchmark, launch forward, volume, pose and joint VJP once each (profiling driver)",
+    )
     parser.add_argument(
         "--copy-calibration",
         type=float,
@@ -70,6 +75,38 @@ def copy_calibration(gib: float) -> int:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_per_launch": count * 4, "launches": 3}), flush=True)
+    return 0
+
+
+def kernels_once(args) -> int:
+    """One launch each of the forward and the volume-only, pose-only and joint
+    VJP kernels on the benchmark workload, through the C ABI (profiling driver)."""
+    import numpy as np
+    import torch
+
+    from dpt_amd import _native, phantoms
+
+    grid = phantoms.centred_grid(SHAPE)
+    values = phantoms.chest_phantom(grid, seed=2026)
+    geometry, _ = phantoms.circular_geometry(grid, args.views, DETECTOR, sod=SOD, sdd=SDD)
+    panels = np.asarray([p.packed() for p in geometry.views])
+    field = torch.tensor(values, device="cuda:0")
+    poses = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float64, device="cuda:0")
+    poses = poses.expand(args.views, 12).contiguous()
+    out = torch.empty((args.views, *DETECTOR), device="cuda:0")
+    grad_field = torch.empty(grid.shape, device="cuda:0")
+    grad_pose = torch.empty((args.views, 12), dtype=torch.float64, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    plan = _native.Plan(grid, views=args.views, rows=DETECTOR[0], cols=DETECTOR[1], panels=panels, rays=None,
+                        samples=args.samples, arithmetic="float32", storage="float32", needs=3, device=0)
+    plan.forward(field.data_ptr(), poses.data_ptr(), out.data_ptr(), stream)
+    seed = (out - 0.5 * out.mean()).contiguous()
+    for flags in (_native.VOLUME, _native.POSE, _native.VOLUME | _native.POSE):
+        plan.vjp(field.data_ptr(), poses.data_ptr(), seed.data_ptr(), grad_field.data_ptr(), grad_pose.data_ptr(),
+                 flags, stream)
+    torch.cuda.synchronize()
+    codes = plan.status(stream)
+    print(json.dumps({"kernels_once": True, "status_max": int(codes.max())}), flush=True)
     return 0
 
 
@@ -115,6 +152,8 @@ def main() -> int:
     args = _args()
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
+    if args.kernels_once:
+        return kernels_once(args)
     if args.gpus != 1:
         print(
             json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (see DESIGN.md)",

@@ -254,19 +254,24 @@ struct Cell32 {
 };
 
 // kernels/projection_fp32.py:64-74 + 130-140. Returns false outside support
-// (NaN and infinities fail the comparisons, as finite_point does).
+// (NaN and infinities fail the comparisons, as finite_point does).  For the
+// finite coordinates that reach the clamp, med3 equals min(max(p, 0), n - 1)
+// and min(floor(v), n - 2) in float equals the reference's integer min.
 __device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, float p2, Cell32 &c) {
     if (!(p0 >= -0.5f && p1 >= -0.5f && p2 >= -0.5f && p0 <= P.hx && p1 <= P.hy && p2 <= P.hz))
         return false;
-    const float v0 = fminf(fmaxf(p0, 0.0f), P.cx);
-    const float v1 = fminf(fmaxf(p1, 0.0f), P.cy);
-    const float v2 = fminf(fmaxf(p2, 0.0f), P.cz);
-    c.lx = min((int)floorf(v0), P.capx);
-    c.ly = min((int)floorf(v1), P.capy);
-    c.lz = min((int)floorf(v2), P.capz);
-    c.fx = v0 - (float)c.lx;
-    c.fy = v1 - (float)c.ly;
-    c.fz = v2 - (float)c.lz;
+    const float v0 = __builtin_amdgcn_fmed3f(p0, 0.0f, P.cx);
+    const float v1 = __builtin_amdgcn_fmed3f(p1, 0.0f, P.cy);
+    const float v2 = __builtin_amdgcn_fmed3f(p2, 0.0f, P.cz);
+    const float f0 = fminf(floorf(v0), (float)P.capx);
+    const float f1 = fminf(floorf(v1), (float)P.capy);
+    const float f2 = fminf(floorf(v2), (float)P.capz);
+    c.lx = (int)f0;
+    c.ly = (int)f1;
+    c.lz = (int)f2;
+    c.fx = v0 - f0;
+    c.fy = v1 - f1;
+    c.fz = v2 - f2;
     c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
     c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
     c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
@@ -367,6 +372,26 @@ struct Acc<true> {
     __device__ __forceinline__ float value() const { return k.s; }
 };
 
+// Position of sample k and its interpolation cell; c.base = -1 outside the
+// support (kernels/projection_fp32.py:116-127, 130-140).
+__device__ __forceinline__ void sample_at(const Params &P, const float *o, const float *d, float lower, float step,
+                                          int k, float &pos, Cell32 &c) {
+    pos = fmaf((float)k + 0.5f, step, lower);
+    const float p0 = fmaf(pos, d[0], o[0]);
+    const float p1 = fmaf(pos, d[1], o[1]);
+    const float p2 = fmaf(pos, d[2], o[2]);
+    if (!cell32(P, p0, p1, p2, c)) c.base = -1;
+}
+
+// The reference flags any non-finite sample point (projection_fp32.py:366-367).
+// Sample positions are monotone in the sample index, so every point of samples
+// [k0, k1) is finite iff the first and the last are.
+__device__ __forceinline__ bool finite_span(const float *o, const float *d, float lower, float step, int k0, int k1) {
+    const float a = fmaf((float)k0 + 0.5f, step, lower), b = fmaf((float)(k1 - 1) + 0.5f, step, lower);
+    return finite3f(fmaf(a, d[0], o[0]), fmaf(a, d[1], o[1]), fmaf(a, d[2], o[2])) &&
+           finite3f(fmaf(b, d[0], o[0]), fmaf(b, d[1], o[1]), fmaf(b, d[2], o[2]));
+}
+
 // ------------------------------------------------------- FP32 forward ------
 // kernels/projection_fp32.py:334-374
 
@@ -399,21 +424,23 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
     if (iv.upper > iv.lower) {
         const float step = (iv.upper - iv.lower) / (float)P.samples;
         bool bad = false;
+        // Lockstep sample walk with the current cell's corners cached in
+        // registers: only lanes whose cell changed issue the (masked) loads.
+        bad = !finite_span(o, d, iv.lower, step, 0, P.samples);
+        float v[8];
+        int cur = -1;
         for (int k = 0; k < P.samples; ++k) {
-            const float pos = fmaf((float)k + 0.5f, step, iv.lower);
-            const float p0 = fmaf(pos, d[0], o[0]);
-            const float p1 = fmaf(pos, d[1], o[1]);
-            const float p2 = fmaf(pos, d[2], o[2]);
-            bad |= !finite3f(p0, p1, p2);
+            float pos;
             Cell32 c;
-            float value = 0.0f;
-            if (cell32(P, p0, p1, p2, c)) {
-                float wt[8], v[8];
-                weights32(c, wt);
+            sample_at(P, o, d, iv.lower, step, k, pos, c);
+            if (c.base < 0) continue;
+            if (c.base != cur) {
                 load8<T, PAIR>(P, field, c.base, v);
-                value = interpolate32(wt, v);
+                cur = c.base;
             }
-            total.add(value);
+            float wt[8];
+            weights32(c, wt);
+            total.add(interpolate32(wt, v));
         }
         if (bad) flag(P, view, DPT_STATUS_RANGE);
         result = total.value() * (len * step);
@@ -516,6 +543,46 @@ __device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, float *b
     cs.reset();
 }
 
+// Cell transition with the shared-face carry of the reference
+// (kernels/projection_fp32.py:272-331): when the new cell is the old one
+// stepped by +-1 along one axis, the four corners on the shared face stay in
+// registers (renamed to the opposite side of the new cell) and only the four
+// departing corners are flushed; any other move flushes all eight.  Selects
+// instead of branches keep the wave converged; a flushed lane with a zero sum
+// issues no LDS atomic.
+template <bool LDS>
+__device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
+                                           float *brick, double *acc) {
+    if (cs.x >= 0) {
+        const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
+        const int ax = (dx != 0) + (dy != 0) + (dz != 0) == 1 && abs(dx + dy + dz) == 1
+                           ? (dx != 0 ? 0 : (dy != 0 ? 1 : 2))
+                           : -1;
+        const int bit = ax < 0 ? 0 : (1 << ax);
+        const int keep = (dx + dy + dz) > 0 ? bit : 0;  // corners with this bit value are shared
+        float carried[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool shared = ax >= 0 && (k & bit) == keep;
+            carried[k] = shared ? cs.s[k] : 0.0f;
+            if (shared) cs.s[k] = 0.0f;
+        }
+        if constexpr (LDS)
+            flush_lds(B, cs, brick);
+        else
+            flush_global(P, cs, acc);
+        // renamed: corner k of the old cell is corner k ^ bit of the new one
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float from1 = carried[j ^ 1], from2 = carried[j ^ 2], from4 = carried[j ^ 4];
+            cs.s[j] = bit == 1 ? from1 : (bit == 2 ? from2 : (bit == 4 ? from4 : 0.0f));
+        }
+    }
+    cs.x = c.lx;
+    cs.y = c.ly;
+    cs.z = c.lz;
+}
+
 constexpr int kChunk = 32;         // samples per lockstep chunk
 constexpr int kBrickCap = 8192;    // LDS floats per workgroup (32 KiB)
 
@@ -552,9 +619,6 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     int view, row, col, wslot;
     bool wave_valid;
     if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
-    double gradient[kMaxComponents];
-#pragma unroll
-    for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
     const double *pose = P.poses + 12 * (size_t)view;
 
     // ---- per-ray setup
@@ -655,39 +719,60 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
                 const long long vol = (long long)e[0] * e[1] * e[2];
                 if (vol <= kBrickCap) {
+                    // every lane holds the same values: keep them in SGPRs
                     use_lds = true;
-                    B.x0 = l[0];
-                    B.y0 = l[1];
-                    B.z0 = l[2];
-                    B.bx = e[0];
-                    B.by = e[1];
-                    B.bz = e[2];
+                    B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
+                    B.y0 = __builtin_amdgcn_readfirstlane(l[1]);
+                    B.z0 = __builtin_amdgcn_readfirstlane(l[2]);
+                    B.bx = __builtin_amdgcn_readfirstlane(e[0]);
+                    B.by = __builtin_amdgcn_readfirstlane(e[1]);
+                    B.bz = __builtin_amdgcn_readfirstlane(e[2]);
                     B.lsx = P.nx > 1 ? 1 : 0;
-                    B.lsy = P.ny > 1 ? e[0] : 0;
-                    B.lsz = P.nz > 1 ? e[0] * e[1] : 0;
-                    B.volume = (int)vol;
+                    B.lsy = P.ny > 1 ? B.bx : 0;
+                    B.lsz = P.nz > 1 ? B.bx * B.by : 0;
+                    B.volume = B.bx * B.by * B.bz;
                 }
             }
         }
         if (active) {
+            // Lockstep walk of this chunk with the current cell cached (see
+            // forward32_kernel); cell changes trigger masked loads and the
+            // shared-face transition of the register sums.
+            bad |= !finite_span(o, d, iv.lower, step, k0, k1);
+            float v[8];
+            int cur = -1;
             for (int k = k0; k < k1; ++k) {
-                const float pos = fmaf((float)k + 0.5f, step, iv.lower);
-                const float p0 = fmaf(pos, d[0], o[0]);
-                const float p1 = fmaf(pos, d[1], o[1]);
-                const float p2 = fmaf(pos, d[2], o[2]);
-                bad |= !finite3f(p0, p1, p2);
+                float pos;
                 Cell32 c;
-                const bool inside = cell32(P, p0, p1, p2, c);
-                float wt[8];
-                if (inside) weights32(c, wt);
-                if constexpr (POSE) {
-                    float value = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
-                    if (inside) {
-                        float v[8];
-                        load8<T, PAIR>(P, field, c.base, v);
-                        value = interpolate32(wt, v);
-                        slopes32(c, v, g);
+                sample_at(P, o, d, iv.lower, step, k, pos, c);
+                if (c.base < 0) {  // outside support: nothing to add, close the cell
+                    if constexpr (VOL) {
+                        if (use_lds)
+                            flush_lds(B, cs, brick);
+                        else
+                            flush_global(P, cs, acc);
                     }
+                    cur = -1;
+                    continue;
+                }
+                if (c.base != cur) {
+                    if constexpr (POSE) load8<T, PAIR>(P, field, c.base, v);
+                    if constexpr (VOL) {
+                        if (bounded) {
+                            if (use_lds)
+                                transition<true>(P, B, cs, c, brick, acc);
+                            else
+                                transition<false>(P, B, cs, c, brick, acc);
+                        }
+                    }
+                    cur = c.base;
+                }
+                float wt[8];
+                weights32(c, wt);
+                if constexpr (POSE) {
+                    float g[3];
+                    const float value = interpolate32(wt, v);
+                    slopes32(c, v, g);
                     mean.add(value);
                     dor0.add(g[0]);
                     dor1.add(g[1]);
@@ -715,31 +800,16 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     }
                 }
                 if constexpr (VOL) {
-                    if (inside) {
-                        if (bounded) {
-                            if (c.lx != cs.x || c.ly != cs.y || c.lz != cs.z) {
-                                if (use_lds)
-                                    flush_lds(B, cs, brick);
-                                else
-                                    flush_global(P, cs, acc);
-                                cs.x = c.lx;
-                                cs.y = c.ly;
-                                cs.z = c.lz;
-                            }
+                    if (bounded) {
 #pragma unroll
-                            for (int kk = 0; kk < 8; ++kk) cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
-                        } else {
-#pragma unroll
-                            for (int kk = 0; kk < 8; ++kk) {
-                                const float inc = scatter_increment(volume_factor, wt[kk]);
-                                if (!isfinite(inc)) bad = true;
-                                if (inc != 0.0f) unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
-                            }
-                        }
-                    } else if (use_lds) {
-                        flush_lds(B, cs, brick);
+                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
                     } else {
-                        flush_global(P, cs, acc);
+#pragma unroll
+                        for (int kk = 0; kk < 8; ++kk) {
+                            const float inc = scatter_increment(volume_factor, wt[kk]);
+                            if (!isfinite(inc)) bad = true;
+                            if (inc != 0.0f) unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
+                        }
                     }
                 }
             }
@@ -774,6 +844,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     }
     if (bad) flag(P, view, DPT_STATUS_RANGE);
     if constexpr (POSE) {
+        double gradient[kMaxComponents];
+#pragma unroll
+        for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
         if (active) {
             const float m = mean.value();
             const float lower_seed = fmaf(span, shl.value(), -m);
