@@ -30,6 +30,10 @@ EXPORTS = (
     "dpt_count_hit_rays",
     "dpt_last_error",
     "dpt_abi_version",
+    "dpt_fit_create",
+    "dpt_fit_destroy",
+    "dpt_fit_step",
+    "dpt_fit_last_error",
 )
 
 ARITH_FP32, ARITH_FP64 = 0, 1

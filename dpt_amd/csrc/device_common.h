@@ -1,0 +1,412 @@
+// Device-side building blocks shared by the projector and the pose-fit
+// kernels: launch parameters, FP64 ray setup, FP32 clipping, trilinear
+// sampling, slopes, scatter increments and the FP64 pose epilogue.  Every
+// function restates the reference operation for operation (citations inline).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <climits>
+#include <cstdint>
+#include <type_traits>
+
+#include "../../include/dpt_amd.h"
+
+namespace dpt {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 16;
+constexpr int kXcds = 8;
+constexpr int kMaxComponents = 12;
+
+struct Params {
+    int nx, ny, nz;
+    int sx, sy, sz;          // corner strides: 1, nx, nx*ny, or 0 for singleton axes
+    float hx, hy, hz;        // n - 0.5 (upper support bound)
+    float cx, cy, cz;        // n - 1 (upper clamp)
+    int capx, capy, capz;    // max(n - 2, 0)
+    double g[9];             // object_to_grid
+    double go[3];            // grid origin
+    int views, rows, cols, samples;
+    int bundle;              // 0: panels, 1: explicit rays
+    int vb;                  // views per workgroup (1, 2 or 4): one wave64 per view when > 1
+    int tile_w, tile_h;      // pixels per workgroup tile of one view
+    int tiles_x, tiles_y, tiles_per_group, total_tiles, chunk;
+    int wt_x, wt_y, wt_per_view;  // wave tiles (8x8 pixels, or 64 rays) per view
+    const double *panels;    // views x 12
+    const double *ray_src;   // views x pixels x 3
+    const double *ray_end;
+    const double *poses;     // views x 12
+    int *status;             // views
+};
+
+// ------------------------------------------------------------------ utils --
+
+__device__ __forceinline__ void flag(const Params &P, int view, int bits) {
+    atomicOr(&P.status[view], bits);
+}
+
+__device__ __forceinline__ bool finite3f(float a, float b, float c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+__device__ __forceinline__ bool finite3d(const double *v) {
+    return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
+}
+
+// Workgroup -> rays.  Consecutive workgroup ids land on different XCDs (round
+// robin); the remap hands each XCD a contiguous run of the tile walk (view
+// group, tile row, tile column) so the tiles an XCD has in flight sample a
+// compact part of the volume and share its L2.  A workgroup covers `vb`
+// consecutive views: vb = 1 is a 16x16 pixel tile (four 8x8 wave quadrants),
+// vb = 2 an 8x16 tile of two views, vb = 4 an 8x8 tile of four views (one wave
+// per view).  Each wave owns one 8x8 wave tile of one view; `wslot` indexes its
+// pose partials.  Explicit ray bundles use 256 consecutive rays per workgroup.
+// Returns false when the whole workgroup lies past the end of the walk.
+__device__ __forceinline__ bool lane_ray(const Params &P, int &view, int &row, int &col, int &wslot,
+                                         bool &wave_valid) {
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * P.chunk + b / kXcds;
+    if (logical >= P.total_tiles) return false;
+    const int group = logical / P.tiles_per_group;
+    const int r = logical - group * P.tiles_per_group;
+    const int ty = r / P.tiles_x;
+    const int tx = r - ty * P.tiles_x;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    if (P.bundle) {
+        view = group;
+        row = 0;
+        col = tx * kBlock + t;
+        const int wx = tx * (kBlock / 64) + w;
+        wslot = view * P.wt_per_view + wx;
+        wave_valid = wx < P.wt_x;
+        return true;
+    }
+    int vw, qy, qx;
+    if (P.vb == 1) {
+        vw = 0;
+        qy = w >> 1;
+        qx = w & 1;
+    } else if (P.vb == 2) {
+        vw = w >> 1;
+        qy = 0;
+        qx = w & 1;
+    } else {
+        vw = w;
+        qy = 0;
+        qx = 0;
+    }
+    view = group * P.vb + vw;
+    const int wy = ty * (P.tile_h >> 3) + qy, wx = tx * (P.tile_w >> 3) + qx;
+    row = wy * 8 + (l >> 3);
+    col = wx * 8 + (l & 7);
+    wslot = view * P.wt_per_view + wy * P.wt_x + wx;
+    wave_valid = view < P.views && wy < P.wt_y && wx < P.wt_x;
+    return true;
+}
+
+__device__ __forceinline__ bool ray_in_range(const Params &P, int view, int row, int col) {
+    return view < P.views && row < P.rows && col < P.cols;
+}
+
+// ------------------------------------------------------------- ray setup --
+// kernels/projection.py:79-102 in FP64; same operation order as the oracle.
+
+struct World {
+    double wsr[3], wd[3], os[3], od[3];
+};
+
+__device__ __forceinline__ void ray_world(const Params &P, const double *pose, int view, int row,
+                                          int col, World &w) {
+    double src[3], end[3];
+    if (!P.bundle) {
+        const double *pn = P.panels + 12 * (size_t)view;
+        for (int i = 0; i < 3; ++i) {
+            src[i] = pn[i];
+            end[i] = pn[3 + i] + (double)col * pn[6 + i] + (double)row * pn[9 + i];
+        }
+    } else {
+        const size_t ray = (size_t)view * P.cols + col;
+        for (int i = 0; i < 3; ++i) {
+            src[i] = P.ray_src[3 * ray + i];
+            end[i] = P.ray_end[3 * ray + i];
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        w.wsr[i] = src[i] - pose[9 + i];
+        w.wd[i] = end[i] - src[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        w.os[i] = pose[i] * w.wsr[0] + pose[3 + i] * w.wsr[1] + pose[6 + i] * w.wsr[2];
+        w.od[i] = pose[i] * w.wd[0] + pose[3 + i] * w.wd[1] + pose[6 + i] * w.wd[2];
+    }
+}
+
+struct Ray {
+    double o[3], d[3], len;
+    bool valid;
+};
+
+__device__ __forceinline__ void ray_grid(const Params &P, const World &w, Ray &r) {
+    double rel[3];
+    for (int i = 0; i < 3; ++i) rel[i] = w.os[i] - P.go[i];
+    for (int i = 0; i < 3; ++i) {
+        r.o[i] = P.g[3 * i] * rel[0] + P.g[3 * i + 1] * rel[1] + P.g[3 * i + 2] * rel[2];
+        r.d[i] = P.g[3 * i] * w.od[0] + P.g[3 * i + 1] * w.od[1] + P.g[3 * i + 2] * w.od[2];
+    }
+    r.len = sqrt(w.wd[0] * w.wd[0] + w.wd[1] * w.wd[1] + w.wd[2] * w.wd[2]);
+    // kernels/projection.py:110-121 (valid_ray)
+    r.valid = finite3d(r.o) && finite3d(r.d) && finite3d(w.os) && finite3d(w.od) &&
+              finite3d(w.wsr) && finite3d(w.wd) && isfinite(r.len) && r.len > 0.0;
+}
+
+// --------------------------------------------------------- FP32 sampling --
+
+struct Interval32 {
+    float lower, upper;
+    int lo_axis, up_axis;    // active clip face (-1: none)
+    float lo_o, lo_d, up_o, up_d;
+};
+
+// kernels/projection_fp32.py:77-109; only the single active face derivative
+// is non-zero, so the interval keeps its axis and two scalars per end.
+__device__ __forceinline__ Interval32 interval32(const Params &P, const float *o, const float *d) {
+    Interval32 iv;
+    iv.lower = 0.0f;
+    iv.upper = 1.0f;
+    iv.lo_axis = iv.up_axis = -1;
+    iv.lo_o = iv.lo_d = iv.up_o = iv.up_d = 0.0f;
+    const float highs[3] = {P.hx, P.hy, P.hz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float low = -0.5f, high = highs[a], v = d[a];
+        if (v == 0.0f) {
+            if (o[a] < low || o[a] > high) iv.upper = -1.0f;
+        } else {
+            float first = (low - o[a]) / v, last = (high - o[a]) / v;
+            if (first > last) {
+                const float t = first;
+                first = last;
+                last = t;
+            }
+            if (first > iv.lower) {
+                iv.lower = first;
+                iv.lo_axis = a;
+                iv.lo_o = -1.0f / v;
+                iv.lo_d = -first / v;
+            }
+            if (last < iv.upper) {
+                iv.upper = last;
+                iv.up_axis = a;
+                iv.up_o = -1.0f / v;
+                iv.up_d = -last / v;
+            }
+        }
+    }
+    return iv;
+}
+
+template <typename T>
+__device__ __forceinline__ float load_value(const T *f, int i);
+template <>
+__device__ __forceinline__ float load_value<float>(const float *f, int i) {
+    return f[i];
+}
+template <>
+__device__ __forceinline__ float load_value<__hip_bfloat16>(const __hip_bfloat16 *f, int i) {
+    const uint16_t bits = reinterpret_cast<const uint16_t *>(f)[i];
+    return __uint_as_float((uint32_t)bits << 16);
+}
+template <>
+__device__ __forceinline__ float load_value<__half>(const __half *f, int i) {
+    return __half2float(f[i]);
+}
+
+struct Cell32 {
+    int base;                // linear index of the lower corner
+    int lx, ly, lz;          // lower corner
+    float fx, fy, fz;        // fractions
+    float slx, sly, slz;     // slopes (0/1)
+};
+
+// kernels/projection_fp32.py:64-74 + 130-140. Returns false outside support
+// (NaN and infinities fail the comparisons, as finite_point does).  For the
+// finite coordinates that reach the clamp, med3 equals min(max(p, 0), n - 1)
+// and min(floor(v), n - 2) in float equals the reference's integer min.
+__device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, float p2, Cell32 &c) {
+    if (!(p0 >= -0.5f && p1 >= -0.5f && p2 >= -0.5f && p0 <= P.hx && p1 <= P.hy && p2 <= P.hz))
+        return false;
+    const float v0 = __builtin_amdgcn_fmed3f(p0, 0.0f, P.cx);
+    const float v1 = __builtin_amdgcn_fmed3f(p1, 0.0f, P.cy);
+    const float v2 = __builtin_amdgcn_fmed3f(p2, 0.0f, P.cz);
+    const float f0 = fminf(floorf(v0), (float)P.capx);
+    const float f1 = fminf(floorf(v1), (float)P.capy);
+    const float f2 = fminf(floorf(v2), (float)P.capz);
+    c.lx = (int)f0;
+    c.ly = (int)f1;
+    c.lz = (int)f2;
+    c.fx = v0 - f0;
+    c.fy = v1 - f1;
+    c.fz = v2 - f2;
+    c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
+    c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
+    c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
+    c.base = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+    return true;
+}
+
+// corner k value offsets: bit0 x, bit1 y, bit2 z (kernels/projection.py:217-222;
+// lower <= n-2 so only singleton axes clamp, which a zero stride reproduces).
+__device__ __forceinline__ int corner_offset(const Params &P, int k) {
+    return ((k & 1) ? P.sx : 0) + ((k & 2) ? P.sy : 0) + ((k & 4) ? P.sz : 0);
+}
+
+// corner weights (f0*f1)*f2 (kernels/projection_fp32.py:143-154)
+__device__ __forceinline__ void weights32(const Cell32 &c, float *w) {
+    const float ax = 1.0f - c.fx, ay = 1.0f - c.fy, az = 1.0f - c.fz;
+    const float xy[4] = {ax * ay, c.fx * ay, ax * c.fy, c.fx * c.fy};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = xy[k & 3] * ((k & 4) ? c.fz : az);
+}
+
+// Eight corner values.  With PAIR (FP32 storage, nx > 1) each x-adjacent pair
+// (lower x, lower x + 1 <= nx - 1) is one dword-aligned 8-byte load: four
+// loads per sample instead of eight, never past the end of an x row.
+struct __attribute__((packed, aligned(4))) FloatPair {
+    float a, b;
+};
+
+template <typename T, bool PAIR>
+__device__ __forceinline__ void load8(const Params &P, const T *field, int base, float *v) {
+    if constexpr (PAIR && std::is_same<T, float>::value) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const FloatPair q =
+                *reinterpret_cast<const FloatPair *>(field + base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0));
+            v[2 * j] = q.a;
+            v[2 * j + 1] = q.b;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = load_value<T>(field, base + corner_offset(P, k));
+    }
+}
+
+// sum_k fma(w_k, v_k, .) in corner order (kernels/projection_fp32.py:165-168)
+__device__ __forceinline__ float interpolate32(const float *w, const float *v) {
+    float value = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) value = fmaf(w[k], v[k], value);
+    return value;
+}
+
+// kernels/projection_fp32.py:196-208 (differences before interpolation)
+__device__ __forceinline__ void slopes32(const Cell32 &c, const float *v, float *g) {
+    const float fx = c.fx, fy = c.fy, fz = c.fz;
+    const float ax = 1.0f - fx, ay = 1.0f - fy, az = 1.0f - fz;
+    const float x0 = fmaf(fy, v[3] - v[2], ay * (v[1] - v[0]));
+    const float x1 = fmaf(fy, v[7] - v[6], ay * (v[5] - v[4]));
+    g[0] = c.slx * fmaf(fz, x1, az * x0);
+    const float y0 = fmaf(fx, v[3] - v[1], ax * (v[2] - v[0]));
+    const float y1 = fmaf(fx, v[7] - v[5], ax * (v[6] - v[4]));
+    g[1] = c.sly * fmaf(fz, y1, az * y0);
+    const float z0 = fmaf(fx, v[5] - v[1], ax * (v[4] - v[0]));
+    const float z1 = fmaf(fx, v[7] - v[3], ax * (v[6] - v[2]));
+    g[2] = c.slz * fmaf(fy, z1, ay * z0);
+}
+
+// kernels/projection_fp32.py:240-252: RN(factor*weight), |x| < 2^-126 -> signed 0
+__device__ __forceinline__ float scatter_increment(float factor, float weight) {
+    const float x = factor * weight;
+    return fabsf(x) < 0x1p-126f ? copysignf(0.0f, x) : x;
+}
+
+// Compensated (Kahan) accumulator, used only by half-storage kernels.
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void init() { s = c = 0.0f; }
+    __device__ __forceinline__ void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+template <bool COMPENSATED>
+struct Acc {
+    float s;
+    __device__ __forceinline__ void init() { s = 0.0f; }
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ float value() const { return s; }
+};
+template <>
+struct Acc<true> {
+    Kahan k;
+    __device__ __forceinline__ void init() { k.init(); }
+    __device__ __forceinline__ void add(float x) { k.add(x); }
+    __device__ __forceinline__ float value() const { return k.s; }
+};
+
+// Position of sample k and its interpolation cell; c.base = -1 outside the
+// support (kernels/projection_fp32.py:116-127, 130-140).
+__device__ __forceinline__ void sample_at(const Params &P, const float *o, const float *d, float lower, float step,
+                                          int k, float &pos, Cell32 &c) {
+    pos = fmaf((float)k + 0.5f, step, lower);
+    const float p0 = fmaf(pos, d[0], o[0]);
+    const float p1 = fmaf(pos, d[1], o[1]);
+    const float p2 = fmaf(pos, d[2], o[2]);
+    if (!cell32(P, p0, p1, p2, c)) c.base = -1;
+}
+
+// The reference flags any non-finite sample point (projection_fp32.py:366-367).
+// Sample positions are monotone in the sample index, so every point of samples
+// [k0, k1) is finite iff the first and the last are.
+__device__ __forceinline__ bool finite_span(const float *o, const float *d, float lower, float step, int k0, int k1) {
+    const float a = fmaf((float)k0 + 0.5f, step, lower), b = fmaf((float)(k1 - 1) + 0.5f, step, lower);
+    return finite3f(fmaf(a, d[0], o[0]), fmaf(a, d[1], o[1]), fmaf(a, d[2], o[2])) &&
+           finite3f(fmaf(b, d[0], o[0]), fmaf(b, d[1], o[1]), fmaf(b, d[2], o[2]));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// FP64 pose epilogue: kernels/projection_fp32.py:562-602 / projection.py:1156-1217
+template <int C>
+__device__ __forceinline__ void pose_epilogue(const Params &P, const double *pose, const World &w,
+                                              double factor, const double *og, const double *dg,
+                                              double *out) {
+    double sa[3], da[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double s = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+        const double t = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+        sa[i] = factor * s;
+        da[i] = factor * t;
+    }
+    if constexpr (C == 12) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[3 * r + c] = w.wsr[r] * sa[c] + w.wd[r] * da[c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            out[9 + a] = -(pose[3 * a] * sa[0] + pose[3 * a + 1] * sa[1] + pose[3 * a + 2] * sa[2]);
+    } else {
+        const double *os = w.os, *od = w.od;
+        out[0] = -sa[0];
+        out[1] = -sa[1];
+        out[2] = -sa[2];
+        out[3] = (sa[1] * os[2] - sa[2] * os[1]) + (da[1] * od[2] - da[2] * od[1]);
+        out[4] = (sa[2] * os[0] - sa[0] * os[2]) + (da[2] * od[0] - da[0] * od[2]);
+        out[5] = (sa[0] * os[1] - sa[1] * os[0]) + (da[0] * od[1] - da[1] * od[0]);
+    }
+}
+
+
+}  // namespace dpt

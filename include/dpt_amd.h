@@ -145,6 +145,64 @@ int64_t dpt_count_hit_rays(dpt_plan *plan, const double *poses, void *stream);
 const char *dpt_last_error(void);
 int32_t dpt_abi_version(void);
 
+/* ------------------------------------------------------------------------
+ * Pose-fit service (batched solves on resident volumes).  No reference
+ * counterpart: the reference runs one solve per process
+ * (python/dpt/pose_optimization.py:753-1006, python/dpt/registration.py).
+ * ---------------------------------------------------------------------- */
+
+/* A resident volume: device field in DPT_STORE_* storage plus its grid. */
+typedef struct {
+    const void *field;
+    int32_t storage;
+    int32_t nx, ny, nz;
+    double object_to_grid[9];
+    double grid_origin[3];
+} dpt_volume_desc;
+
+/* One active solve: `views` cone panels (device float64 (views, 12), as for
+ * dpt_plan_create), measured optical depth `targets` (device float32
+ * (views, rows, cols)) and its device solver state (DPT_FIT_STATE_SIZE
+ * float64, layout below; the caller initialises the pose and zeroes the
+ * rest). */
+typedef struct {
+    int32_t volume; /* index into the volume table passed with the step */
+    int32_t views, rows, cols, samples;
+    const double *panels;
+    const float *targets;
+    double *state;
+} dpt_fit_request;
+
+#define DPT_FIT_POSE 0        /* 12: pose evaluated by the next step */
+#define DPT_FIT_BEST_POSE 12  /* 12: lowest-loss pose so far (the result) */
+#define DPT_FIT_BEST_LOSS 24  /* 1/2 sum of squared residuals at BEST_POSE */
+#define DPT_FIT_LAMBDA 25     /* Levenberg-Marquardt damping */
+#define DPT_FIT_EVALS 26      /* evaluations done */
+#define DPT_FIT_ACCEPTED 27   /* accepted steps */
+#define DPT_FIT_JTJ 28        /* 21: J^T J (upper, row-major) at BEST_POSE */
+#define DPT_FIT_JTR 49        /* 6: J^T r at BEST_POSE */
+#define DPT_FIT_LAST_LOSS 55  /* loss of the last evaluation */
+#define DPT_FIT_STATE_SIZE 56
+
+typedef struct dpt_fit dpt_fit;
+
+/* Allocate the device tables for up to max_requests (<= 256) solves per step
+ * and max_tiles 16x16 detector tiles in total. */
+int dpt_fit_create(int32_t device, int32_t max_requests, int64_t max_tiles, dpt_fit **out);
+int dpt_fit_destroy(dpt_fit *fit);
+
+/* Advance every listed solve by one Levenberg-Marquardt iteration: one FP32
+ * march per ray gives the optical depth (residual against the target) and the
+ * local-right pose Jacobian row; normal equations are summed in FP64 in a
+ * fixed order per request and solved on the device.  With use_graph != 0 the
+ * step runs as a captured hipGraph (one per grid size and stream; the stream
+ * must not be the legacy default stream).  No host synchronisation. */
+int dpt_fit_step(dpt_fit *fit, const dpt_volume_desc *volumes, int32_t n_volumes,
+                 const dpt_fit_request *requests, int32_t n_requests, int32_t use_graph,
+                 void *stream);
+const char *dpt_fit_last_error(void);
+
+
 #ifdef __cplusplus
 }
 #endif
