@@ -48,6 +48,9 @@ def _args():
     parser.add_argument("--storage", default="float32", choices=["float32", "bfloat16", "float16"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-rows-stride", type=int, default=1)
+    parser.add_argument("--no-serving", action="store_true", help="skip the batched pose-fit serving measurement")
+    parser.add_argument("--serving-requests", type=int, default=32)
+    parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
     parser.add_argument(
         "--kernels-once",
         action="store_true",
@@ -108,6 +111,100 @@ def kernels_once(args) -> int:
     codes = plan.status(stream)
     print(json.dumps({"kernels_once": True, "status_max": int(codes.max())}), flush=True)
     return 0
+
+
+def serving_workload(count: int, seed: int = 404):
+    """Pose-fit workload: 16 resident seeded phantoms with shapes drawn from
+    {128^3 at 1 mm, 256^3 at 1 mm, 512x507x76 at 0.45x0.45x2.5 mm}; each
+    request has 2-8 cone views at random angles, a 128^2/256^2/512^2 detector,
+    256/512/768/1024 samples per ray, 20-120 iterations and a target rendered
+    by the FP32 projector at a random pose (+/-10 deg, +/-20 mm) plus noise."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from dpt_amd import phantoms, serving
+    from dpt_amd.geometry import ConeBeamIntrinsics, RigidTransform, compose_pose
+    from dpt_amd.torch import Projector
+
+    rng = np.random.default_rng(seed)
+    shapes = [((128, 128, 128), (1.0, 1.0, 1.0)), ((256, 256, 256), (1.0, 1.0, 1.0)), ((76, 507, 512), (0.45, 0.45, 2.5))]
+    volumes = []
+    for i in range(16):
+        shape, spacing = shapes[int(rng.integers(3))]
+        grid = phantoms.centred_grid(shape, spacing)
+        volumes.append((f"vol{i:02d}", grid, phantoms.chest_phantom(grid, seed=100 + i)))
+    requests, truths = [], []
+    for j in range(count):
+        key, grid, values = volumes[int(rng.integers(16))]
+        views = int(rng.integers(2, 9))
+        det = int(rng.choice([128, 256, 512]))
+        samples = int(rng.choice([256, 512, 768, 1024]))
+        iterations = int(rng.integers(20, 121))
+        angles = rng.uniform(0.0, 2 * math.pi, views)
+        xi = np.concatenate([rng.uniform(-20, 20, 3), np.radians(rng.uniform(-10, 10, 3))])
+        nx, ny, nz = grid.xyz
+        diag = math.hypot(nx * grid.spacing_mm[0], ny * grid.spacing_mm[1], nz * grid.spacing_mm[2])
+        sod = max(1000.0, 2.0 * diag)
+        sdd = 1.5 * sod
+        pitch = phantoms.coverage_pitch(grid, sod, sdd, angles, (det, det), margin=1.15)
+        geometry = ConeBeamIntrinsics(sod, sdd, (det, det), (pitch, pitch), centre_mm=grid.centre_mm()).batch(angles)
+        truth = compose_pose(RigidTransform(), tuple(xi)).packed()
+        with torch.no_grad():
+            image = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)(
+                torch.tensor(values, device="cuda:0"), packed_pose=torch.tensor(truth, dtype=torch.float64, device="cuda:0")
+            )
+            image += 1e-3 * torch.randn(image.shape, generator=torch.Generator("cuda:0").manual_seed(j), device="cuda:0")
+        requests.append(serving.FitRequest(f"req{j:03d}", key, geometry, image.cpu().numpy(), samples, iterations))
+        truths.append(truth)
+    return volumes, requests, truths
+
+
+def serving_benchmark(count: int, rate: float) -> dict:
+    """Batched BF16 service (continuous batching, hipGraph steps) versus the
+    plain per-request FP32 form (one solve at a time, direct launches), both
+    closed-loop (all requests queued) and open-loop (Poisson arrivals)."""
+    import dataclasses
+
+    import numpy as np
+
+    from dpt_amd import serving
+
+    volumes, requests, truths = serving_workload(count)
+    arrivals = serving.poisson_arrivals(count, rate, seed=405)
+    timed = [dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, arrivals)]
+    out = {"requests": count, "arrival_rate_per_s": rate}
+    forms = {"batched_bf16": ("bfloat16", 64, True), "plain_fp32": ("float32", 1, False)}
+    for name, (storage, slots, graph) in forms.items():
+        registry = serving.VolumeRegistry()
+        for key, grid, values in volumes:
+            registry.add(key, values, grid, storage=storage)
+        engine = serving.PoseFitEngine(registry, max_requests=slots, max_tiles=1 << 19, use_graph=graph)
+        engine.run(requests[:2], realtime=False)  # warm-up: code objects, graphs
+        closed = engine.run(requests, realtime=False)
+        opened = engine.run(timed, realtime=True)
+        errors = []
+        for result in closed.results:
+            k = int(result.key[3:])
+            errors.append(serving.pose_error(result.pose, truths[k]))
+        errors = np.asarray(errors)
+        out[name] = {
+            "storage": storage,
+            "slots": slots,
+            "graph": graph,
+            "closed_loop": closed.summary(),
+            "open_loop": opened.summary(),
+            "median_rotation_error_deg": round(float(np.median(errors[:, 0])), 5),
+            "median_translation_error_mm": round(float(np.median(errors[:, 1])), 5),
+        }
+        engine.close()
+        _log(f"serving {name}: {out[name]['closed_loop']['solves_per_s']} solves/s closed-loop, "
+             f"p50/p99 {out[name]['open_loop']['p50_latency_ms']}/{out[name]['open_loop']['p99_latency_ms']} ms")
+    out["throughput_ratio_closed_loop"] = round(
+        out["batched_bf16"]["closed_loop"]["solves_per_s"] / out["plain_fp32"]["closed_loop"]["solves_per_s"], 3
+    )
+    return out
 
 
 def _log(message: str) -> None:
@@ -285,6 +382,11 @@ def main() -> int:
         },
         "reference_blackwell_iteration_gsamples": 65.5,
     }
+    if not args.no_serving:
+        _log("serving benchmark")
+        result["serving"] = serving_benchmark(args.serving_requests, args.serving_rate)
+        result["p50_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p50_latency_ms"]
+        result["p99_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p99_latency_ms"]
     if not args.no_cpu_baseline:
         _log("timing CPU oracle baseline")
         result["cpu_baseline"] = cpu_baseline(grid, geometry, values, args.samples, args.cpu_rows_stride)

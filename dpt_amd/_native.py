@@ -41,6 +41,8 @@ STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
 
 STORAGE_CODES = {"float32": STORE_F32, "bfloat16": STORE_BF16, "float16": STORE_F16}
+# 16-bit field buffers carry DPT_HALF_PAD_BYTES (8) of readable padding
+HALF_PAD_ELEMENTS = 4
 ARITH_CODES = {"float32": ARITH_FP32, "float64": ARITH_FP64}
 
 

@@ -277,6 +277,20 @@ __device__ __forceinline__ void weights32(const Cell32 &c, float *w) {
 struct __attribute__((packed, aligned(4))) FloatPair {
     float a, b;
 };
+struct __attribute__((packed, aligned(4))) UintPair {
+    uint32_t a, b;
+};
+
+template <typename T>
+__device__ __forceinline__ float half_bits_to_float(uint32_t bits);
+template <>
+__device__ __forceinline__ float half_bits_to_float<__hip_bfloat16>(uint32_t bits) {
+    return __uint_as_float(bits << 16);
+}
+template <>
+__device__ __forceinline__ float half_bits_to_float<__half>(uint32_t bits) {
+    return __half2float(__ushort_as_half((unsigned short)bits));
+}
 
 template <typename T, bool PAIR>
 __device__ __forceinline__ void load8(const Params &P, const T *field, int base, float *v) {
@@ -287,6 +301,20 @@ __device__ __forceinline__ void load8(const Params &P, const T *field, int base,
                 *reinterpret_cast<const FloatPair *>(field + base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0));
             v[2 * j] = q.a;
             v[2 * j + 1] = q.b;
+        }
+    } else if constexpr (PAIR) {
+        // 16-bit storage: the x-adjacent pair (i, i+1) lies in the dword-aligned
+        // 8 bytes at dword i/2; one dwordx2 load and a funnel shift extract it.
+        // Reads at most 4 bytes past element i+1: half-storage buffers carry
+        // DPT_HALF_PAD_BYTES of padding.
+        const uint32_t *words = reinterpret_cast<const uint32_t *>(field);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0);
+            const UintPair q = *reinterpret_cast<const UintPair *>(words + (i >> 1));
+            const uint32_t pair = __builtin_amdgcn_alignbit(q.b, q.a, (i & 1) * 16);
+            v[2 * j] = half_bits_to_float<T>(pair & 0xFFFFu);
+            v[2 * j + 1] = half_bits_to_float<T>(pair >> 16);
         }
     } else {
 #pragma unroll
@@ -335,19 +363,41 @@ struct Kahan {
     }
 };
 
+// Along-ray accumulator.  Plain: one FP32 running sum, the reference's
+// arithmetic.  Compensated (half-storage forms): blocked compensated
+// summation -- an FP32 partial sum over each block of samples, folded into a
+// Kahan-compensated total at block boundaries -- which keeps the error of a
+// 1024-sample ray near that of full Kahan at ~1 extra add per sample.
+constexpr int kFoldBlock = 32;
+
 template <bool COMPENSATED>
 struct Acc {
     float s;
     __device__ __forceinline__ void init() { s = 0.0f; }
     __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ void fma_add(float a, float b) { s = fmaf(a, b, s); }
+    __device__ __forceinline__ void fold() {}
     __device__ __forceinline__ float value() const { return s; }
 };
 template <>
 struct Acc<true> {
+    float blk;
     Kahan k;
-    __device__ __forceinline__ void init() { k.init(); }
-    __device__ __forceinline__ void add(float x) { k.add(x); }
-    __device__ __forceinline__ float value() const { return k.s; }
+    __device__ __forceinline__ void init() {
+        blk = 0.0f;
+        k.init();
+    }
+    __device__ __forceinline__ void add(float x) { blk += x; }
+    __device__ __forceinline__ void fma_add(float a, float b) { blk = fmaf(a, b, blk); }
+    __device__ __forceinline__ void fold() {
+        k.add(blk);
+        blk = 0.0f;
+    }
+    __device__ __forceinline__ float value() const {
+        Kahan t = k;
+        t.add(blk);
+        return t.s;
+    }
 };
 
 // Position of sample k and its interpolation cell; c.base = -1 outside the

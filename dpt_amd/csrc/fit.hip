@@ -55,7 +55,9 @@ struct FitTable {
 };
 
 // Ray march of one request ray: L and the 6-component unit-seed Jacobian row.
-template <typename T>
+// FP32 storage sums plainly (the projector's FP32 VJP arithmetic); 16-bit
+// storage uses blocked compensated sums.
+template <typename T, bool COMPENSATED>
 __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const double *pose, int view, int row,
                                         int col, float &L, double *j, bool &valid) {
     World w;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
     if (!(iv.upper > iv.lower)) return;
     const float span = iv.upper - iv.lower;
     const float step = span / (float)P.samples;
-    Kahan mean, dor0, dor1, dor2, shl, shu, dd0, dd1, dd2;
+    Acc<COMPENSATED> mean, dor0, dor1, dor2, shl, shu, dd0, dd1, dd2;
     mean.init();
     dor0.init();
     dor1.init();
@@ -87,19 +89,28 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
     float v[8];
     int cur = -1;
     for (int k = 0; k < P.samples; ++k) {
+        if constexpr (COMPENSATED) {
+            if (k > 0 && (k % kFoldBlock) == 0) {
+                mean.fold();
+                dor0.fold();
+                dor1.fold();
+                dor2.fold();
+                shl.fold();
+                shu.fold();
+                dd0.fold();
+                dd1.fold();
+                dd2.fold();
+            }
+        }
         float pos;
         Cell32 c;
         sample_at(P, o, d, iv.lower, step, k, pos, c);
         if (c.base < 0) continue;
         if (c.base != cur) {
-            if constexpr (std::is_same<T, float>::value) {
-                if (P.sx)
-                    load8<T, true>(P, field, c.base, v);
-                else
-                    load8<T, false>(P, field, c.base, v);
-            } else {
+            if (P.sx)
+                load8<T, true>(P, field, c.base, v);
+            else
                 load8<T, false>(P, field, c.base, v);
-            }
             cur = c.base;
         }
         float wt[8], g[3];
@@ -109,19 +120,20 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
         dor0.add(g[0]);
         dor1.add(g[1]);
         dor2.add(g[2]);
-        dd0.add(pos * g[0]);
-        dd1.add(pos * g[1]);
-        dd2.add(pos * g[2]);
+        dd0.fma_add(pos, g[0]);
+        dd1.fma_add(pos, g[1]);
+        dd2.fma_add(pos, g[2]);
         const float alpha = ((float)k + 0.5f) / (float)P.samples;
         const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
-        shl.add((1.0f - alpha) * along);
-        shu.add(alpha * along);
+        shl.fma_add(1.0f - alpha, along);
+        shu.fma_add(alpha, along);
     }
-    const float m = mean.s;
+    const float m = mean.value();
     L = m * (len * step);
-    const float lower_seed = fmaf(span, shl.s, -m);
-    const float upper_seed = fmaf(span, shu.s, m);
-    const float dor[3] = {dor0.s, dor1.s, dor2.s}, ddir[3] = {dd0.s, dd1.s, dd2.s};
+    const float lower_seed = fmaf(span, shl.value(), -m);
+    const float upper_seed = fmaf(span, shu.value(), m);
+    const float dor[3] = {dor0.value(), dor1.value(), dor2.value()};
+    const float ddir[3] = {dd0.value(), dd1.value(), dd2.value()};
     double og[3], dg[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -172,12 +184,12 @@ __global__ __launch_bounds__(kBlock) void fit_step_kernel(const FitTable *__rest
             double jr[6];
             bool valid;
             if (R.storage == DPT_STORE_BF16)
-                fit_ray<__hip_bfloat16>(P, static_cast<const __hip_bfloat16 *>(R.field), R.state, view, row, col, L, jr,
-                                        valid);
+                fit_ray<__hip_bfloat16, true>(P, static_cast<const __hip_bfloat16 *>(R.field), R.state, view, row,
+                                              col, L, jr, valid);
             else if (R.storage == DPT_STORE_F16)
-                fit_ray<__half>(P, static_cast<const __half *>(R.field), R.state, view, row, col, L, jr, valid);
+                fit_ray<__half, true>(P, static_cast<const __half *>(R.field), R.state, view, row, col, L, jr, valid);
             else
-                fit_ray<float>(P, static_cast<const float *>(R.field), R.state, view, row, col, L, jr, valid);
+                fit_ray<float, false>(P, static_cast<const float *>(R.field), R.state, view, row, col, L, jr, valid);
             const double res = (double)L - (double)R.targets[((size_t)view * P.rows + row) * P.cols + col];
             int q = 0;
 #pragma unroll

@@ -78,6 +78,9 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
         float v[8];
         int cur = -1;
         for (int k = 0; k < P.samples; ++k) {
+            if constexpr (COMPENSATED) {
+                if (k > 0 && (k % kFoldBlock) == 0) total.fold();
+            }
             float pos;
             Cell32 c;
             sample_at(P, o, d, iv.lower, step, k, pos, c);
@@ -266,12 +269,14 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         }
     }
 
-    Acc<COMPENSATED> mean, dor0, dor1, dor2, shl, shu;
-    float ddir[3] = {0.0f, 0.0f, 0.0f}, ddc[3] = {0.0f, 0.0f, 0.0f};
+    Acc<COMPENSATED> mean, dor0, dor1, dor2, dd0, dd1, dd2, shl, shu;
     mean.init();
     dor0.init();
     dor1.init();
     dor2.init();
+    dd0.init();
+    dd1.init();
+    dd2.init();
     shl.init();
     shu.init();
     CellSums cs;
@@ -387,27 +392,14 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     dor0.add(g[0]);
                     dor1.add(g[1]);
                     dor2.add(g[2]);
-                    if constexpr (COMPENSATED) {
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const float y = fmaf(pos, g[a], -ddc[a]);
-                            const float t = ddir[a] + y;
-                            ddc[a] = (t - ddir[a]) - y;
-                            ddir[a] = t;
-                        }
-                    } else {
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) ddir[a] = fmaf(pos, g[a], ddir[a]);
-                    }
+                    // contracted by the reference's code generator (fuse_fp)
+                    dd0.fma_add(pos, g[0]);
+                    dd1.fma_add(pos, g[1]);
+                    dd2.fma_add(pos, g[2]);
                     const float alpha = ((float)k + 0.5f) / (float)P.samples;
                     const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
-                    if constexpr (COMPENSATED) {
-                        shl.add((1.0f - alpha) * along);
-                        shu.add(alpha * along);
-                    } else {
-                        shl.s = fmaf(1.0f - alpha, along, shl.s);
-                        shu.s = fmaf(alpha, along, shu.s);
-                    }
+                    shl.fma_add(1.0f - alpha, along);
+                    shu.fma_add(alpha, along);
                 }
                 if constexpr (VOL) {
                     if (bounded) {
@@ -423,6 +415,17 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     }
                 }
             }
+        }
+        if constexpr (POSE && COMPENSATED) {
+            mean.fold();
+            dor0.fold();
+            dor1.fold();
+            dor2.fold();
+            dd0.fold();
+            dd1.fold();
+            dd2.fold();
+            shl.fold();
+            shu.fold();
         }
         if constexpr (VOL) {
             if (use_lds) {
@@ -462,6 +465,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             const float lower_seed = fmaf(span, shl.value(), -m);
             const float upper_seed = fmaf(span, shu.value(), m);
             const float dor[3] = {dor0.value(), dor1.value(), dor2.value()};
+            const float ddir[3] = {dd0.value(), dd1.value(), dd2.value()};
             double og[3], dg[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1029,14 +1033,11 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, h
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         float *o = static_cast<float *>(out);
-        if constexpr (std::is_same<T, float>::value) {
-            if (P.nx > 1)
-                hipLaunchKernelGGL((forward32_kernel<T, false, true>), grid, block, 0, s, P, f, o);
-            else
-                hipLaunchKernelGGL((forward32_kernel<T, false, false>), grid, block, 0, s, P, f, o);
-        } else {
-            hipLaunchKernelGGL((forward32_kernel<T, true, false>), grid, block, 0, s, P, f, o);
-        }
+        constexpr bool HALF = !std::is_same<T, float>::value;
+        if (P.nx > 1)
+            hipLaunchKernelGGL((forward32_kernel<T, HALF, true>), grid, block, 0, s, P, f, o);
+        else
+            hipLaunchKernelGGL((forward32_kernel<T, HALF, false>), grid, block, 0, s, P, f, o);
     } else {
         hipLaunchKernelGGL((forward64_kernel<T>), grid, block, 0, s, P, f, static_cast<double *>(out));
     }
@@ -1066,14 +1067,10 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         const float *sd = static_cast<const float *>(seeds);
-        if constexpr (std::is_same<T, float>::value) {
-            if (P.nx > 1)
-                launch_vjp32<T, C, true>(p, P, f, sd, vol, pose, s);
-            else
-                launch_vjp32<T, C, false>(p, P, f, sd, vol, pose, s);
-        } else {
+        if (P.nx > 1)
+            launch_vjp32<T, C, true>(p, P, f, sd, vol, pose, s);
+        else
             launch_vjp32<T, C, false>(p, P, f, sd, vol, pose, s);
-        }
     } else {
         const double *sd = static_cast<const double *>(seeds);
         if (vol && pose)

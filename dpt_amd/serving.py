@@ -21,7 +21,7 @@ import math
 import threading
 import time
 from collections import deque
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -130,7 +130,7 @@ class VolumeRegistry:
         if storage == "float32":
             tensor = src
         else:
-            tensor = torch.empty(src.numel(), dtype=getattr(torch, storage), device=self.device)
+            tensor = torch.empty(src.numel() + _native.HALF_PAD_ELEMENTS, dtype=getattr(torch, storage), device=self.device)
             stream = torch.cuda.current_stream(self.device).cuda_stream
             _native.convert_field(src.data_ptr(), tensor.data_ptr(), src.numel(), storage, None, stream)
         # resident data must be complete before any engine stream reads it
@@ -342,6 +342,7 @@ class PoseFitEngine:
         results: list[FitResult] = []
         used = 0
         steps = peak = peak_bytes = 0
+        inflight: deque = deque()
         start = time.perf_counter()
         with self._lock, torch.cuda.stream(self.stream):
             while pending or queued or active:
@@ -368,11 +369,20 @@ class PoseFitEngine:
                     continue
                 self.step(active)
                 steps += 1
-                self.stream.synchronize()
                 for a in active:
                     a.done += 1
                 finished = [a for a in active if a.done >= a.request.iterations]
+                # Host and device overlap: synchronise only when a solve ends;
+                # otherwise keep at most two steps in flight (the native table
+                # staging is double-buffered).
+                event = torch.cuda.Event()
+                event.record(self.stream)
+                inflight.append(event)
+                if not finished and len(inflight) > 2:
+                    inflight.popleft().synchronize()
                 if finished:
+                    self.stream.synchronize()
+                    inflight.clear()
                     stop = time.perf_counter() - start
                     states = torch.stack([a.state for a in finished]).cpu().numpy()
                     for a, st in zip(finished, states):

@@ -56,7 +56,7 @@ def _field_pointer(entry, field: torch.Tensor, stream: int) -> int:
     if storage == "float32":
         return field.data_ptr()
     dtype = torch.bfloat16 if storage == "bfloat16" else torch.float16
-    half = torch.empty(field.numel(), dtype=dtype, device=field.device)
+    half = torch.empty(field.numel() + _native.HALF_PAD_ELEMENTS, dtype=dtype, device=field.device)
     _native.convert_field(field.data_ptr(), half.data_ptr(), field.numel(), storage, None, stream)
     entry.half = half  # keep alive until the next call on this plan
     return half.data_ptr()

@@ -45,6 +45,9 @@ extern "C" {
 #define DPT_STORE_F32 0
 #define DPT_STORE_BF16 1
 #define DPT_STORE_F16 2
+/* BF16/FP16 field buffers must stay readable for this many bytes past the
+ * last voxel (x-adjacent pairs are fetched with one 8-byte load). */
+#define DPT_HALF_PAD_BYTES 8
 
 /* dpt_vjp flags */
 #define DPT_VOLUME 1u        /* produce grad_field */

@@ -55,7 +55,7 @@ class NativeScene:
         if self.storage == "float32":
             return f, f
         dtype = torch.bfloat16 if self.storage == "bfloat16" else torch.float16
-        half = torch.empty(f.numel(), dtype=dtype, device="cuda")
+        half = torch.empty(f.numel() + _native.HALF_PAD_ELEMENTS, dtype=dtype, device="cuda")
         _native.convert_field(f.data_ptr(), half.data_ptr(), f.numel(), self.storage, None, self.stream)
         return half, f
 
