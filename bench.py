@@ -51,6 +51,9 @@ def _args():
     parser.add_argument("--no-serving", action="store_true", help="skip the batched pose-fit serving measurement")
     parser.add_argument("--serving-requests", type=int, default=32)
     parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
+    parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
+    parser.add_argument("--transport-histories", type=int, default=100_000_000)
+    parser.add_argument("--transport-cpu-histories", type=int, default=4_000_000)
     parser.add_argument(
         "--kernels-once",
         action="store_true",
@@ -207,6 +210,67 @@ def serving_benchmark(count: int, rate: float) -> dict:
     return out
 
 
+def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
+    """Monte Carlo transport: 128^3 three-material chest label map, AP cone-beam
+    point source, 256^2 primary/scatter tallies, 60 keV and a 40-120 keV
+    spectrum.  Histories/s of simulate() (source sampling, Woodcock walk,
+    tallies, moments, one status readback) and of the trace kernel alone."""
+    import torch
+
+    from dpt_amd import transport
+
+    out = {"workload": "128^3 chest labels (air/soft tissue/bone at 0.0012/1.0/1.85 g/cm^3), 1 mm, "
+                       "SOD/SDD 1000/1500 mm, 256^2 detector, Klein-Nishina + Rayleigh, Woodcock majorant",
+           "histories": histories}
+    for name, spectrum in (("mono_60kev", False), ("spectrum_40_120kev", True)):
+        spec, physical, labels, a, s, dens, source = transport.chest_transport_scene(spectrum=spectrum)
+        ws = transport.prepare_transport(spec, material_ids=labels, absorption=a, scattering=s,
+                                         max_histories=histories, physical=physical)
+        sim = transport.prepare_simulation(ws)
+        transport.simulate(dens, source=source, batch=transport.HistoryBatch(1, 0, min(1 << 20, histories)), workspace=sim)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record()
+        result = transport.simulate(dens, source=source, batch=transport.HistoryBatch(2026, 0, histories), workspace=sim)
+        stop.record()
+        wall = time.perf_counter() - t0
+        stop.synchronize()
+        gpu_ms = start.elapsed_time(stop)
+        primary = float(result.primary_mean.sum())
+        scatter = float(result.scatter_mean.sum())
+        out[name] = {
+            "histories_per_s": round(histories / wall, 1),
+            "wall_ms": round(wall * 1e3, 3),
+            "gpu_ms": round(gpu_ms, 3),
+            "status_histogram": list(result.status_histogram.counts),
+            "detected_primary_per_history": round(primary, 6),
+            "detected_scatter_per_history": round(scatter, 6),
+            "scatter_to_primary": round(scatter / primary, 5) if primary > 0 else None,
+        }
+        if cpu:
+            from oracle import transport as T
+
+            from dpt_amd.transport.simulation import bind_model
+
+            model = T.Model(*bind_model(spec, material_ids=labels, absorption=a, scattering=s, physical=physical))
+            t0 = time.perf_counter()
+            T.trace(model, dens, source, 2026, 0, cpu_histories, records=False)
+            elapsed = time.perf_counter() - t0
+            out[name]["cpu_baseline"] = {
+                "value": round(cpu_histories / elapsed, 1),
+                "unit": "histories/s",
+                "cores": T.threads(),
+                "kind": "port",
+                "sample": f"CPU oracle walk (OpenMP over histories), first {cpu_histories} histories of the same "
+                          f"seed, tallies included: {elapsed:.2f} s",
+            }
+        _log(f"transport {name}: {out[name]['histories_per_s']:.4g} histories/s "
+             f"({out[name]['wall_ms']:.1f} ms for {histories})")
+        ws.close()
+    return out
+
+
 def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -253,7 +317,7 @@ def main() -> int:
         return kernels_once(args)
     if args.gpus != 1:
         print(
-            json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+            json.dumps({"error": "this build measures one GPU; sharding across GPUs is described, not run",
                         "n_gpus": args.gpus}),
             flush=True,
         )
@@ -387,6 +451,10 @@ def main() -> int:
         result["serving"] = serving_benchmark(args.serving_requests, args.serving_rate)
         result["p50_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p50_latency_ms"]
         result["p99_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p99_latency_ms"]
+    if not args.no_transport:
+        _log("transport benchmark")
+        result["transport"] = transport_benchmark(args.transport_histories, args.transport_cpu_histories,
+                                                  not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         _log("timing CPU oracle baseline")
         result["cpu_baseline"] = cpu_baseline(grid, geometry, values, args.samples, args.cpu_rows_stride)

@@ -34,6 +34,12 @@ EXPORTS = (
     "dpt_fit_destroy",
     "dpt_fit_step",
     "dpt_fit_last_error",
+    "dpt_transport_create",
+    "dpt_transport_destroy",
+    "dpt_transport_trace",
+    "dpt_transport_moments",
+    "dpt_transport_status",
+    "dpt_transport_last_error",
 )
 
 ARITH_FP32, ARITH_FP64 = 0, 1
@@ -57,6 +63,78 @@ class GridDesc(ctypes.Structure):
         ("nz", ctypes.c_int32),
         ("object_to_grid", ctypes.c_double * 9),
         ("grid_origin", ctypes.c_double * 3),
+    ]
+
+
+class TransportModel(ctypes.Structure):
+    """dpt_transport_model (include/dpt_amd.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("materials", ctypes.c_int32),
+        ("energy_bins", ctypes.c_int32),
+        ("spectrum_bins", ctypes.c_int32),
+        ("max_events", ctypes.c_int32),
+        ("max_crossings", ctypes.c_int32),
+        ("max_angle_trials", ctypes.c_int32),
+        ("compton", ctypes.c_int32),
+        ("compton_packed", ctypes.c_int32),
+        ("energy_score", ctypes.c_int32),
+        ("physics", ctypes.c_int32),
+        ("woodcock", ctypes.c_int32),
+        ("det_shape", ctypes.c_int32 * 2),
+        ("origin", ctypes.c_double * 3),
+        ("spacing", ctypes.c_double * 3),
+        ("det_corner", ctypes.c_double * 3),
+        ("det_u", ctypes.c_double * 3),
+        ("det_v", ctypes.c_double * 3),
+        ("det_normal", ctypes.c_double * 3),
+        ("det_spacing", ctypes.c_double * 2),
+        ("inv_rot", ctypes.c_double * 9),
+        ("inv_trans", ctypes.c_double * 3),
+        ("source_energy", ctypes.c_double),
+        ("cutoff", ctypes.c_double),
+        ("material_ids", ctypes.c_void_p),
+        ("energies", ctypes.c_void_p),
+        ("absorption", ctypes.c_void_p),
+        ("scattering", ctypes.c_void_p),
+        ("rayleigh", ctypes.c_void_p),
+        ("majorant", ctypes.c_void_p),
+        ("majorant_max_density", ctypes.c_void_p),
+        ("q_offsets", ctypes.c_void_p),
+        ("q", ctypes.c_void_p),
+        ("form_factor", ctypes.c_void_p),
+        ("atomic_number", ctypes.c_void_p),
+        ("spectrum_energies", ctypes.c_void_p),
+        ("spectrum_cdf", ctypes.c_void_p),
+    ]
+
+
+class PointSource(ctypes.Structure):
+    """dpt_point_source."""
+
+    _fields_ = [
+        ("source", ctypes.c_double * 3),
+        ("corner", ctypes.c_double * 3),
+        ("u", ctypes.c_double * 3),
+        ("v", ctypes.c_double * 3),
+        ("extent", ctypes.c_double * 2),
+        ("weight", ctypes.c_double),
+    ]
+
+
+class HistoryRecords(ctypes.Structure):
+    """dpt_history_records (device pointers, each optional)."""
+
+    _fields_ = [
+        ("pixel", ctypes.c_void_p),
+        ("score", ctypes.c_void_p),
+        ("energy", ctypes.c_void_p),
+        ("events", ctypes.c_void_p),
+        ("real_events", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
     ]
 
 
@@ -104,13 +182,29 @@ def lib():
         h.dpt_last_error.restype = ctypes.c_char_p
         h.dpt_abi_version.argtypes = []
         h.dpt_abi_version.restype = i32
+        u64, dbl = ctypes.c_uint64, ctypes.c_double
+        h.dpt_transport_create.argtypes = [ctypes.POINTER(TransportModel), i32, ctypes.POINTER(vp)]
+        h.dpt_transport_create.restype = ctypes.c_int
+        h.dpt_transport_destroy.argtypes = [vp]
+        h.dpt_transport_destroy.restype = ctypes.c_int
+        h.dpt_transport_trace.argtypes = [
+            vp, vp, ctypes.POINTER(PointSource), u64, u64, i64, dbl, ctypes.POINTER(HistoryRecords), vp, vp,
+        ]
+        h.dpt_transport_trace.restype = ctypes.c_int
+        h.dpt_transport_moments.argtypes = [vp, vp, i64, vp, vp, vp]
+        h.dpt_transport_moments.restype = ctypes.c_int
+        h.dpt_transport_status.argtypes = [vp, vp, vp, i32, vp]
+        h.dpt_transport_status.restype = ctypes.c_int
+        h.dpt_transport_last_error.argtypes = []
+        h.dpt_transport_last_error.restype = ctypes.c_char_p
         _LIB = h
         return h
 
 
-def check(rc: int, what: str) -> None:
+def check(rc: int, what: str, error=None) -> None:
     if rc != 0:
-        message = lib().dpt_last_error().decode(errors="replace")
+        source = lib().dpt_last_error if error is None else error
+        message = source().decode(errors="replace")
         if rc == -1:
             raise ContractError(f"{what}: {message}")
         raise NativeLibraryError(f"{what} failed ({rc}): {message}")

@@ -205,6 +205,106 @@ int dpt_fit_step(dpt_fit *fit, const dpt_volume_desc *volumes, int32_t n_volumes
                  void *stream);
 const char *dpt_fit_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Monte Carlo photon transport (analogue Woodcock tracking).
+ *
+ * Replaces the reference's prepare_transport / simulate pair
+ * (python/dpt/transport/forward.py:146-264, transport/simulation.py:186-316)
+ * with the walk of transport/kernels.py:941-1563 and the point-cone source of
+ * transport/kernels.py:2339-2366.  Everything is binary64; random numbers are
+ * the reference's counter-addressed Philox4x32-10 (kernels/random.py:27-60),
+ * so a history's path depends only on (seed, history identity).
+ * ---------------------------------------------------------------------- */
+
+#define DPT_TRANSPORT_MAX_MATERIALS 8
+#define DPT_TRANSPORT_MAX_TABLE_BYTES 49152 /* all tables, staged in LDS */
+
+/* Terminal history statuses (transport/model.py:36-45). */
+#define DPT_HISTORY_ESCAPED 0
+#define DPT_HISTORY_ABSORBED 1
+#define DPT_HISTORY_EVENT_BUDGET 2
+#define DPT_HISTORY_CROSSING_BUDGET 3
+#define DPT_HISTORY_NUMERICAL_FAILURE 4
+#define DPT_HISTORY_ENERGY_SUPPORT 5
+#define DPT_HISTORY_ANGLE_BUDGET 6
+
+/* Host-side model; every table pointer is host memory, copied at create.
+ * Field meanings follow the reference's Parameters / PhysicalParameters
+ * structs (transport/kernels.py:105-133, physical_kernels.py:20-45). */
+typedef struct dpt_transport_model {
+    int32_t nx, ny, nz, materials, energy_bins, spectrum_bins;
+    int32_t max_events, max_crossings, max_angle_trials;
+    int32_t compton;        /* 0 isotropic elastic, 1 free-electron Klein-Nishina */
+    int32_t compton_packed; /* two proposals per Philox block */
+    int32_t energy_score;   /* score keV instead of photon counts */
+    int32_t physics;        /* Rayleigh channel, spectrum and majorant bound */
+    int32_t woodcock;       /* majorant present (requires physics) */
+    int32_t det_shape[2];   /* width (columns), height (rows) */
+    double origin[3], spacing[3]; /* lower cell faces and cell size, mm */
+    double det_corner[3], det_u[3], det_v[3], det_normal[3], det_spacing[2]; /* object frame */
+    double inv_rot[9], inv_trans[3]; /* world -> object (inverse object pose) */
+    double source_energy, cutoff;    /* keV */
+    const uint8_t *material_ids;     /* nx*ny*nz labels, x fastest */
+    const double *energies;          /* K increasing nodes, keV */
+    const double *absorption;        /* M*K, energy fastest, mm^-1 per unit density */
+    const double *scattering;        /* M*K (coherent + incoherent) */
+    const double *rayleigh;          /* M*K coherent part (physics) */
+    const double *majorant;          /* K (woodcock) */
+    const double *majorant_max_density; /* M: 1 where the material occurs, else 0 */
+    const int32_t *q_offsets;        /* M+1 offsets into q / form_factor (physics) */
+    const double *q, *form_factor;   /* momentum transfer (1/angstrom), F(q) */
+    const double *atomic_number;     /* M */
+    const double *spectrum_energies, *spectrum_cdf; /* spectrum_bins each */
+} dpt_transport_model;
+
+/* Point emission aimed at a uniform point of the detector rectangle
+ * corner + U*extent[0]*u + V*extent[1]*v (transport/source.py:54-182). */
+typedef struct dpt_point_source {
+    double source[3], corner[3], u[3], v[3], extent[2], weight;
+} dpt_point_source;
+
+/* Optional per-history device outputs (trace_histories, kernels.py:1944-1994,
+ * plus the real-interaction count that tells primary from scatter). */
+typedef struct dpt_history_records {
+    int32_t *pixel;
+    double *score;
+    double *energy;
+    int32_t *events;
+    int32_t *real_events;
+    int32_t *status;
+} dpt_history_records;
+
+typedef struct dpt_transport dpt_transport;
+
+/* Validate the model and upload labels (as 8-bit) and tables. */
+int dpt_transport_create(const dpt_transport_model *model, int32_t device, dpt_transport **out);
+int dpt_transport_destroy(dpt_transport *t);
+
+/* Trace histories first_history .. first_history+count-1 (count < 2^31).
+ * density: host array of `materials` scales (rate = density*table).  With a
+ * majorant, every occurring material must satisfy density*(a+s) < M at every
+ * node (transport/kernels.py:1850-1872), else DPT_E_INVALID.  Scores of
+ * detected histories accumulate into tally (device, 4*pixels doubles: primary
+ * sum, primary sum of squares, scatter sum, scatter sum of squares) when it
+ * is non-NULL; records (device pointers, each optional) receive per-history
+ * results.  Terminal-status counts accumulate on the device until read. */
+int dpt_transport_trace(dpt_transport *t, const double *density, const dpt_point_source *source,
+                        uint64_t seed, uint64_t first_history, int64_t count, double amplitude,
+                        const dpt_history_records *records, double *tally, void *stream);
+
+/* Per-pixel mean and variance of the mean over `histories` original histories
+ * (transport/simulation.py:61-66): mean = sum/N, var = (sumsq - sum^2/N) /
+ * (N(N-1)).  Outputs are device arrays of 3*pixels (total, primary, scatter). */
+int dpt_transport_moments(dpt_transport *t, const double *tally, int64_t histories, double *mean,
+                          double *variance, void *stream);
+
+/* Synchronise the stream, read the seven terminal-status counts and the OR of
+ * 1 << status over failed histories (status >= 2, plus 1 << 4 for a
+ * non-finite score) accumulated since the last clear. */
+int dpt_transport_status(dpt_transport *t, int64_t *histogram, int32_t *failure_bits, int32_t clear,
+                         void *stream);
+const char *dpt_transport_last_error(void);
+
 
 #ifdef __cplusplus
 }
