@@ -52,6 +52,7 @@ def _args():
     parser.add_argument("--serving-requests", type=int, default=32)
     parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
     parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
+    parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
     parser.add_argument("--transport-histories", type=int, default=100_000_000)
     parser.add_argument("--transport-cpu-histories", type=int, default=4_000_000)
     parser.add_argument(
@@ -271,6 +272,64 @@ def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
     return out
 
 
+def precision_benchmark(grid, geometry, values, samples_list=(512, 1024)) -> dict:
+    """Precision form of the full scan: BF16 and FP16 field storage (FP32
+    sampling, compensated along-ray sums) against the FP64 path at the same
+    field, pose and cotangent; FP32 storage alongside.  Relative L2 errors of
+    the image, the volume gradient and the 6-DoF pose gradient, and the
+    forward + joint VJP iteration rate."""
+    import numpy as np
+    import torch
+
+    from dpt_amd.torch import Projector
+
+    device = torch.device("cuda:0")
+    field = torch.tensor(values, device=device, requires_grad=True)
+    pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
+    views, (rows, cols) = len(geometry.views), geometry.shape_rows_columns
+
+    def rel(a, b):
+        a = a.detach().double().reshape(-1)
+        b = b.detach().double().reshape(-1)
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    out = {}
+    for samples in samples_list:
+        ref_module = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)
+        image64 = ref_module(field, pose)
+        cotangent = torch.sin(torch.arange(image64.numel(), device=device, dtype=torch.float64) * 0.001).reshape(
+            image64.shape)
+        g64, p64 = torch.autograd.grad(image64, (field, pose), cotangent)
+        torch.cuda.synchronize()
+        row = {}
+        for storage in ("float32", "bfloat16", "float16"):
+            module = Projector(grid=grid, geometry=geometry, precision="fp32", samples_per_ray=samples,
+                               storage=storage)
+            seed32 = cotangent.float()
+            times = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                image = module(field, pose)
+                g, p = torch.autograd.grad(image, (field, pose), seed32)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+            median = sorted(times)[1]
+            row[storage] = {
+                "image_rel_l2": rel(image, image64),
+                "volume_grad_rel_l2": rel(g, g64),
+                "pose_grad_rel_l2": rel(p, p64),
+                "iteration_ms": round(median * 1e3, 3),
+                "gsamples_per_s": round(views * rows * cols * samples / median / 1e9, 3),
+            }
+            _log(f"precision S={samples} {storage}: image {row[storage]['image_rel_l2']:.2e}, volume "
+                 f"{row[storage]['volume_grad_rel_l2']:.2e}, pose {row[storage]['pose_grad_rel_l2']:.2e}, "
+                 f"{row[storage]['gsamples_per_s']} Gsamples/s")
+        out[f"samples_{samples}"] = row
+        del ref_module, image64, g64, p64
+    return out
+
+
 def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -451,6 +510,9 @@ def main() -> int:
         result["serving"] = serving_benchmark(args.serving_requests, args.serving_rate)
         result["p50_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p50_latency_ms"]
         result["p99_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p99_latency_ms"]
+    if not args.no_precision:
+        _log("precision benchmark")
+        result["precision"] = precision_benchmark(grid, geometry, values)
     if not args.no_transport:
         _log("transport benchmark")
         result["transport"] = transport_benchmark(args.transport_histories, args.transport_cpu_histories,

@@ -922,6 +922,7 @@ struct dpt_plan {
     double *d_acc = nullptr;
     unsigned long long *d_count = nullptr;
     int vjp_vb = 1;          // views per workgroup of the VJP launches
+    int forward_vb = 1;      // views per workgroup of the forward launch
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timed[2] = {false, false};
@@ -1191,6 +1192,11 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         const int forced = std::atoi(vb);
         if (!P.bundle && (forced == 1 || forced == 2 || forced == 4)) p->vjp_vb = forced;
     }
+    p->forward_vb = p->vjp_vb;
+    if (const char *vb = std::getenv("DPT_AMD_FORWARD_VIEWS_PER_BLOCK")) {
+        const int forced = std::atoi(vb);
+        if (!P.bundle && (forced == 1 || forced == 2 || forced == 4)) p->forward_vb = forced;
+    }
 
     auto cleanup = [&](int code, const std::string &msg) {
         dpt_plan_destroy(p);
@@ -1237,7 +1243,8 @@ int dpt_forward(dpt_plan *p, const void *field, const double *poses, void *out, 
         return fail(DPT_E_INVALID, "plan, field, poses and out are required");
     DeviceGuard guard(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const Params P = launch_params(p, poses);
+    Params P = launch_params(p, poses);
+    configure_tiles(P, p->forward_vb);
     switch (p->storage) {
     case DPT_STORE_F32:
         hipLaunchKernelGGL(check_field_kernel<float>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -635,7 +636,10 @@ __device__ __forceinline__ void event_step(const Model &m, const Call &c, const 
 
 __global__ void zero_counter_kernel(unsigned long long *counter) { *counter = 0ull; }
 
-__global__ __launch_bounds__(kBlock) void trace_kernel(Model m, Call c) {
+// WAVES: minimum waves per SIMD the register allocation must allow (2 fits
+// the whole walk in registers; 4 trades a few spills for occupancy).
+template <int WAVES>
+__global__ __launch_bounds__(kBlock, WAVES) void trace_kernel(Model m, Call c) {
     extern __shared__ double lds[];
     __shared__ unsigned int hist[8];
     for (int i = threadIdx.x; i < m.table_doubles; i += kBlock) lds[i] = c.tables[i];
@@ -743,7 +747,8 @@ struct dpt_transport {
     unsigned long long *d_histogram;
     int *d_bits;
     unsigned long long *h_histogram;  // pinned: 7 counts + bits
-    int grid;
+    int grid2, grid4;
+    int waves;  // 2 or 4: trace_kernel variant
 };
 
 namespace {
@@ -952,13 +957,16 @@ int dpt_transport_create(const dpt_transport_model *md, int32_t device, dpt_tran
         dpt_transport_destroy(t);
         return tr_fail(DPT_E_DEVICE, "device upload failed");
     }
-    int per_cu = 0, cus = 0;
+    int per_cu2 = 0, per_cu4 = 0, cus = 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_kernel, kBlock,
-                                                     (size_t)m.table_doubles * sizeof(double)) != hipSuccess)
-        per_cu = 0;
-    t->grid = std::max(1, per_cu) * std::max(1, cus);
+    const size_t lds = (size_t)m.table_doubles * sizeof(double);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, trace_kernel<2>, kBlock, lds) != hipSuccess) per_cu2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu4, trace_kernel<4>, kBlock, lds) != hipSuccess) per_cu4 = 0;
+    t->grid2 = std::max(1, per_cu2) * std::max(1, cus);
+    t->grid4 = std::max(1, per_cu4) * std::max(1, cus);
+    t->waves = 4;
+    if (const char *env = std::getenv("DPT_AMD_MC_WAVES")) t->waves = std::atoi(env) == 2 ? 2 : 4;
     *out = t;
     return DPT_OK;
 }
@@ -1022,9 +1030,14 @@ int dpt_transport_trace(dpt_transport *t, const double *density, const dpt_point
     DeviceGuard guard(t->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t waves = (count + 63) / 64;
-    const int blocks = (int)std::min<int64_t>(t->grid, (waves + 3) / 4);
+    const int grid = t->waves == 2 ? t->grid2 : t->grid4;
+    const int blocks = (int)std::min<int64_t>(grid, (waves + 3) / 4);
+    const size_t lds = (size_t)m.table_doubles * sizeof(double);
     zero_counter_kernel<<<1, 1, 0, s>>>(t->d_counter);
-    trace_kernel<<<blocks, kBlock, (size_t)m.table_doubles * sizeof(double), s>>>(m, c);
+    if (t->waves == 2)
+        trace_kernel<2><<<blocks, kBlock, lds, s>>>(m, c);
+    else
+        trace_kernel<4><<<blocks, kBlock, lds, s>>>(m, c);
     if (hipGetLastError() != hipSuccess) return tr_fail(DPT_E_DEVICE, "trace launch failed");
     return DPT_OK;
 }
