@@ -53,6 +53,8 @@ def _args():
     parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
     parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
     parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
+    parser.add_argument("--transport-only", action="store_true",
+                        help="only the Monte Carlo transport measurement (profiling driver)")
     parser.add_argument("--transport-histories", type=int, default=100_000_000)
     parser.add_argument("--transport-cpu-histories", type=int, default=4_000_000)
     parser.add_argument(
@@ -336,44 +338,87 @@ def _log(message: str) -> None:
 
 def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
     """Time the CPU oracle (FP32 restatement) on views {0,90,180,270}, every
-    ``stride``-th detector row, full columns and samples: forward + joint VJP."""
+    ``stride``-th detector row, full columns and samples: forward + joint VJP,
+    on all host threads; plus the same per-sample work on one thread (view 0,
+    every 16th row) and the small CPU-runnable case (64^3, AP + lateral,
+    128^2 at 0.8 mm, 256 samples, central-region mean loss) in full."""
+    import math
+
     import numpy as np
 
+    from dpt_amd import phantoms
+    from dpt_amd.geometry import ConeBeamIntrinsics
     from oracle import oracle as O
 
+    def run(picks, row_stride, scene_grid, scene_values, geom, s, seed_fn):
+        panels = []
+        for k in picks:
+            p = geom.views[k]
+            rows = (p.shape[0] + row_stride - 1) // row_stride
+            panels.append(O.Panel(p.source_mm, p.origin_mm, p.u, p.v, (p.spacing_mm[0], p.spacing_mm[1] * row_stride),
+                                  (rows, p.shape[1])))
+        scene = O.Scene(O.Grid(scene_grid.shape, scene_grid.spacing_mm, scene_grid.origin_mm,
+                               scene_grid.orientation), s, panels=panels)
+        t0 = time.perf_counter()
+        image, _ = scene.forward(scene_values)
+        scene.vjp(scene_values, seed_fn(image))
+        elapsed = time.perf_counter() - t0
+        nominal = len(picks) * panels[0].shape[0] * panels[0].shape[1] * s
+        return nominal, elapsed, panels[0].shape
+
+    residual_like = lambda image: (image - 0.5 * image.mean()).astype(np.float32)  # noqa: E731
     picks = [v for v in (0, 90, 180, 270) if v < len(geometry.views)]
-    panels = []
-    for k in picks:
-        p = geometry.views[k]
-        rows = (p.shape[0] + stride - 1) // stride
-        panels.append(O.Panel(p.source_mm, p.origin_mm, p.u, p.v, (p.spacing_mm[0], p.spacing_mm[1] * stride), (rows, p.shape[1])))
-    scene = O.Scene(O.Grid(grid.shape, grid.spacing_mm, grid.origin_mm, grid.orientation), samples, panels=panels)
-    t0 = time.perf_counter()
-    image, _ = scene.forward(values)
-    seeds = (image - 0.5 * image.mean()).astype(np.float32)
-    scene.vjp(values, seeds)
-    elapsed = time.perf_counter() - t0
-    rays = len(picks) * panels[0].shape[0] * panels[0].shape[1]
-    nominal = rays * samples
+    nominal, elapsed, shape = run(picks, stride, grid, values, geometry, samples, residual_like)
+    threads = O.threads()
+    O.set_threads(1)
+    try:
+        one_nominal, one_elapsed, one_shape = run([0], 16, grid, values, geometry, samples, residual_like)
+    finally:
+        O.set_threads(threads)
+    # CPU-runnable case: 64^3, AP and lateral, 128^2 at 0.8 mm, S = 256,
+    # loss = mean of the central 64x64 of each view
+    small = phantoms.centred_grid((64, 64, 64))
+    small_values = phantoms.chest_phantom(small, seed=2026)
+    angles = phantoms.standard_angles()
+    small_geometry = ConeBeamIntrinsics(SOD, SDD, (128, 128), (0.8, 0.8), centre_mm=small.centre_mm()).batch(
+        [angles["ap"], angles["lateral"]])
+
+    def central_mean(image):
+        seed = np.zeros(image.shape, dtype=np.float32)
+        seed[:, 32:96, 32:96] = 1.0 / (image.shape[0] * 64 * 64)
+        return seed
+
+    c1_nominal, c1_elapsed, _ = run([0, 1], 1, small, small_values, small_geometry, 256, central_mean)
     return {
         "value": round(nominal / elapsed / 1e9, 6),
         "unit": "Gsamples/s",
-        "cores": O.threads(),
+        "cores": threads,
         "kind": "port",
         "sample": (
             f"CPU oracle (reference-algorithm FP32 restatement, OpenMP over rays) forward + volume + pose VJP on views "
-            f"{picks}, every {stride}th detector row ({panels[0].shape[0]}x{panels[0].shape[1]} rays/view), "
-            f"{samples} samples/ray: {nominal:.3e} nominal samples in {elapsed:.2f} s"
+            f"{picks}, every {stride}th detector row ({shape[0]}x{shape[1]} rays/view), "
+            f"{samples} samples/ray: {nominal:.3e} nominal samples in {elapsed:.2f} s; a full 360-view pass "
+            f"extrapolates to {elapsed * len(geometry.views) / len(picks) * stride:.0f} s"
         ),
+        "single_thread": {
+            "value": round(one_nominal / one_elapsed / 1e9, 6),
+            "unit": "Gsamples/s",
+            "cores": 1,
+            "sample": f"view 0, every 16th row ({one_shape[0]}x{one_shape[1]} rays), {samples} samples/ray: "
+                      f"{one_nominal:.3e} samples in {one_elapsed:.2f} s",
+        },
+        "small_case": {
+            "value": round(c1_nominal / c1_elapsed / 1e9, 6),
+            "unit": "Gsamples/s",
+            "cores": threads,
+            "iteration_ms": round(c1_elapsed * 1e3, 3),
+            "sample": "64^3 phantom, AP + lateral, 128x128 at 0.8 mm, 256 samples, central 64x64 mean loss, in full",
+        },
     }
 
 
 def main() -> int:
     args = _args()
-    if args.copy_calibration > 0:
-        return copy_calibration(args.copy_calibration)
-    if args.kernels_once:
-        return kernels_once(args)
     if args.gpus != 1:
         print(
             json.dumps({"error": "this build measures one GPU; sharding across GPUs is described, not run",
@@ -381,6 +426,14 @@ def main() -> int:
             flush=True,
         )
         return 2
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
+    if args.kernels_once:
+        return kernels_once(args)
+    if args.transport_only:
+        print(json.dumps(transport_benchmark(args.transport_histories, args.transport_cpu_histories,
+                                             not args.no_cpu_baseline)), flush=True)
+        return 0
     import numpy as np
     import torch
 

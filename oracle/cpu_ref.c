@@ -722,3 +722,12 @@ int ref_threads(void) {
     return 1;
 #endif
 }
+
+/* Thread count of later parallel regions (CPU-baseline timing at 1 thread). */
+void ref_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -81,12 +81,19 @@ def _lib():
             getattr(lib, name).argtypes = [ctypes.POINTER(_Plan), p, p, p, p, p, p, ctypes.c_int, p]
             getattr(lib, name).restype = None
         lib.ref_threads.restype = ctypes.c_int
+        lib.ref_set_threads.argtypes = [ctypes.c_int]
+        lib.ref_set_threads.restype = None
         _LIB = lib
     return _LIB
 
 
 def threads() -> int:
     return int(_lib().ref_threads())
+
+
+def set_threads(n: int) -> None:
+    """Thread count for later oracle calls in this process."""
+    _lib().ref_set_threads(int(n))
 
 
 @dataclass(frozen=True)
@@ -404,6 +411,3 @@ def relative_l2(actual, expected) -> float:
     diff = float(np.linalg.norm(a - e))
     return diff / denom if denom > 0 else diff
 
-
-def set_threads(count: int) -> None:
-    os.environ["OMP_NUM_THREADS"] = str(int(count))
