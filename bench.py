@@ -274,12 +274,14 @@ def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
     return out
 
 
-def precision_benchmark(grid, geometry, values, samples_list=(512, 1024)) -> dict:
+def precision_benchmark(grid, geometry, values, target, samples_list=(512, 1024)) -> dict:
     """Precision form of the full scan: BF16 and FP16 field storage (FP32
     sampling, compensated along-ray sums) against the FP64 path at the same
-    field, pose and cotangent; FP32 storage alongside.  Relative L2 errors of
-    the image, the volume gradient and the 6-DoF pose gradient, and the
-    forward + joint VJP iteration rate."""
+    field and pose; FP32 storage alongside.  The cotangent is the FP64
+    residual against the seeded target (the squared-error loss of the full
+    scan), the same for every form.  Relative L2 errors of the image, the
+    volume gradient and the 6-DoF pose gradient, and the forward + joint VJP
+    iteration rate."""
     import numpy as np
     import torch
 
@@ -299,8 +301,7 @@ def precision_benchmark(grid, geometry, values, samples_list=(512, 1024)) -> dic
     for samples in samples_list:
         ref_module = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)
         image64 = ref_module(field, pose)
-        cotangent = torch.sin(torch.arange(image64.numel(), device=device, dtype=torch.float64) * 0.001).reshape(
-            image64.shape)
+        cotangent = (image64 - target.double()).detach()
         g64, p64 = torch.autograd.grad(image64, (field, pose), cotangent)
         torch.cuda.synchronize()
         row = {}
@@ -565,7 +566,7 @@ def main() -> int:
         result["p99_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p99_latency_ms"]
     if not args.no_precision:
         _log("precision benchmark")
-        result["precision"] = precision_benchmark(grid, geometry, values)
+        result["precision"] = precision_benchmark(grid, geometry, values, target)
     if not args.no_transport:
         _log("transport benchmark")
         result["transport"] = transport_benchmark(args.transport_histories, args.transport_cpu_histories,

@@ -345,7 +345,9 @@ __device__ __forceinline__ Angular rayleigh_event(const Model &m, const double *
 // State of the photon a lane carries.
 struct Photon {
     double pos[3], dir[3];
-    double energy, checked_energy;
+    double energy, checked_energy, coord_energy;
+    double fraction, majorant;  // energy coordinate cache (valid at coord_energy)
+    int lower;
     uint64_t history;
     int cell[3];
     int events, real_events, status, crossings;
@@ -362,6 +364,7 @@ __device__ __forceinline__ void start(const Model &m, const Call &c, const doubl
     ph.crossings = 0;
     ph.direct = false;
     ph.checked_energy = -1.0;
+    ph.coord_energy = -1.0;
     ph.energy = m.source_energy;
     double u[4];
     random4(c.seed, history, kNamespaceBit, 0u, u);
@@ -450,13 +453,19 @@ __device__ __forceinline__ void event_step(const Model &m, const Call &c, const 
     double draws[4];
     random4(c.seed, ph.history, (uint32_t)ph.events, 0u, draws);
     double residual = -log(draws[0]);
-    int lower;
-    double fraction;
-    energy_coordinate(m, T, ph.energy, lower, fraction);
+    // Energy node, fraction and majorant are pure functions of the energy:
+    // computed once per energy, reused by every null event at that energy.
+    if (ph.coord_energy != ph.energy) {
+        energy_coordinate(m, T, ph.energy, ph.lower, ph.fraction);
+        ph.majorant = (m.physics && m.woodcock) ? table_value(m, T + m.off_maj, 0, ph.lower, ph.fraction) : 0.0;
+        ph.coord_energy = ph.energy;
+    }
+    const int lower = ph.lower;
+    const double fraction = ph.fraction;
     bool collision = false;
     int material = 0;
-    double a_rate = 0.0, s_rate = 0.0, physical_extinction = 0.0, extinction = 0.0, majorant = 0.0;
-    if (m.physics && m.woodcock) majorant = table_value(m, T + m.off_maj, 0, lower, fraction);
+    double a_rate = 0.0, s_rate = 0.0, physical_extinction = 0.0, extinction = 0.0;
+    const double majorant = ph.majorant;
     if (m.physics && m.woodcock && ph.checked_energy != ph.energy) {
         // transport/kernels.py:363-384
         bool allowed = true;

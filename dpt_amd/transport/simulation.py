@@ -47,6 +47,10 @@ def _as_numpy(value, dtype, name: str) -> np.ndarray:
 
 
 def _check_transport(rc: int, what: str) -> None:
+    """Native status -> TransportError for contract violations (DPT_E_INVALID)."""
+    if rc == -1:
+        message = _native.lib().dpt_transport_last_error().decode(errors="replace")
+        raise TransportError(f"{what}: {message}")
     _native.check(rc, what, _native.lib().dpt_transport_last_error)
 
 
