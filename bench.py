@@ -56,7 +56,7 @@ def _args():
     parser.add_argument("--transport-only", action="store_true",
                         help="only the Monte Carlo transport measurement (profiling driver)")
     parser.add_argument("--transport-histories", type=int, default=100_000_000)
-    parser.add_argument("--transport-cpu-histories", type=int, default=4_000_000)
+    parser.add_argument("--transport-cpu-histories", type=int, default=100_000_000)
     parser.add_argument(
         "--kernels-once",
         action="store_true",
