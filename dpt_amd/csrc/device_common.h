@@ -40,7 +40,24 @@ struct Params {
     const double *ray_end;
     const double *poses;     // views x 12
     int *status;             // views
+    int brick_cap;           // FP64 LDS brick entries per workgroup (volume adjoint)
+    const float *alpha;      // samples entries: RN((k + 0.5) / samples), FP32
 };
+
+// alpha_k = RN((k + 0.5) / S) of the pose clip-face shifts
+// (kernels/projection_fp32.py:455-458), tabulated once on the host (IEEE
+// binary32 division, identical to the device's correctly rounded division):
+// the per-sample division is replaced by one load of a uniform address.
+inline void fill_alpha_table(float *out, int samples) {
+    for (int k = 0; k < samples; ++k) out[k] = ((float)k + 0.5f) / (float)samples;
+}
+
+// k is uniform across the wave: read the table through the constant address
+// space so that the load is a scalar (s_load) one.
+__device__ __forceinline__ float sample_alpha(const Params &P, int k) {
+    typedef const float __attribute__((address_space(4))) *constant_t;
+    return ((constant_t)P.alpha)[k];
+}
 
 // ------------------------------------------------------------------ utils --
 
@@ -292,33 +309,57 @@ __device__ __forceinline__ float half_bits_to_float<__half>(uint32_t bits) {
     return __half2float(__ushort_as_half((unsigned short)bits));
 }
 
+// Field buffers always live in global memory.  Loads go through pointers of
+// this type so that a field pointer read from a device table (not known to the
+// compiler as global) still compiles to global_load rather than flat_load.
+template <typename T>
+using global_const = const T __attribute__((address_space(1)));
+
+template <typename T>
+__device__ __forceinline__ float load_global(global_const<T> *f, int i);
+template <>
+__device__ __forceinline__ float load_global<float>(global_const<float> *f, int i) {
+    return f[i];
+}
+template <>
+__device__ __forceinline__ float load_global<__hip_bfloat16>(global_const<__hip_bfloat16> *f, int i) {
+    const uint16_t bits = reinterpret_cast<global_const<uint16_t> *>(f)[i];
+    return __uint_as_float((uint32_t)bits << 16);
+}
+template <>
+__device__ __forceinline__ float load_global<__half>(global_const<__half> *f, int i) {
+    const uint16_t bits = reinterpret_cast<global_const<uint16_t> *>(f)[i];
+    return __half2float(__ushort_as_half(bits));
+}
+
 template <typename T, bool PAIR>
-__device__ __forceinline__ void load8(const Params &P, const T *field, int base, float *v) {
+__device__ __forceinline__ void load8(const Params &P, const T *field_any, int base, float *v) {
+    global_const<T> *field = (global_const<T> *)field_any;
     if constexpr (PAIR && std::is_same<T, float>::value) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const FloatPair q =
-                *reinterpret_cast<const FloatPair *>(field + base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0));
-            v[2 * j] = q.a;
-            v[2 * j + 1] = q.b;
+            global_const<FloatPair> *q = reinterpret_cast<global_const<FloatPair> *>(
+                field + base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0));
+            v[2 * j] = q->a;
+            v[2 * j + 1] = q->b;
         }
     } else if constexpr (PAIR) {
         // 16-bit storage: the x-adjacent pair (i, i+1) lies in the dword-aligned
         // 8 bytes at dword i/2; one dwordx2 load and a funnel shift extract it.
         // Reads at most 4 bytes past element i+1: half-storage buffers carry
         // DPT_HALF_PAD_BYTES of padding.
-        const uint32_t *words = reinterpret_cast<const uint32_t *>(field);
+        global_const<uint32_t> *words = reinterpret_cast<global_const<uint32_t> *>(field);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int i = base + ((j & 1) ? P.sy : 0) + ((j & 2) ? P.sz : 0);
-            const UintPair q = *reinterpret_cast<const UintPair *>(words + (i >> 1));
-            const uint32_t pair = __builtin_amdgcn_alignbit(q.b, q.a, (i & 1) * 16);
+            global_const<UintPair> *q = reinterpret_cast<global_const<UintPair> *>(words + (i >> 1));
+            const uint32_t pair = __builtin_amdgcn_alignbit(q->b, q->a, (i & 1) * 16);
             v[2 * j] = half_bits_to_float<T>(pair & 0xFFFFu);
             v[2 * j + 1] = half_bits_to_float<T>(pair >> 16);
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = load_value<T>(field, base + corner_offset(P, k));
+        for (int k = 0; k < 8; ++k) v[k] = load_global<T>(field, base + corner_offset(P, k));
     }
 }
 

@@ -123,7 +123,7 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
         dd0.fma_add(pos, g[0]);
         dd1.fma_add(pos, g[1]);
         dd2.fma_add(pos, g[2]);
-        const float alpha = ((float)k + 0.5f) / (float)P.samples;
+        const float alpha = sample_alpha(P, k);
         const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
         shl.fma_add(1.0f - alpha, along);
         shu.fma_add(alpha, along);
@@ -353,6 +353,7 @@ struct dpt_fit {
     double *d_partials = nullptr;
     int *d_counter = nullptr;
     std::map<std::pair<int, hipStream_t>, hipGraphExec_t> graphs;
+    std::map<int, float *> alpha;  // sample-fraction tables by samples per ray
 };
 
 namespace {
@@ -417,6 +418,7 @@ int dpt_fit_destroy(dpt_fit *f) {
     (void)hipGetDevice(&previous);
     (void)hipSetDevice(f->device);
     for (auto &kv : f->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : f->alpha) (void)hipFree(kv.second);
     (void)hipFree(f->d_table);
     (void)hipFree(f->d_partials);
     (void)hipFree(f->d_counter);
@@ -490,6 +492,20 @@ int dpt_fit_step(dpt_fit *f, const dpt_volume_desc *volumes, int32_t n_volumes, 
             (void)hipSetDevice(previous);
             return fit_fail(DPT_E_INVALID, "request " + std::to_string(r) + ": " + (why.empty() ? "invalid volume" : why));
         }
+        auto table = f->alpha.find(q.samples);
+        if (table == f->alpha.end()) {
+            std::vector<float> host((size_t)q.samples);
+            fill_alpha_table(host.data(), q.samples);
+            float *dev = nullptr;
+            if (hipMalloc(&dev, sizeof(float) * host.size()) != hipSuccess ||
+                hipMemcpy(dev, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(dev);
+                (void)hipSetDevice(previous);
+                return fit_fail(DPT_E_MEMORY, "sample-fraction table allocation failed");
+            }
+            table = f->alpha.emplace(q.samples, dev).first;
+        }
+        R.P.alpha = table->second;
         R.field = v.field;
         R.storage = v.storage;
         R.targets = q.targets;

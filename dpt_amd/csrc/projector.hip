@@ -145,13 +145,13 @@ struct Brick {
     int volume;
 };
 
-__device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, float *brick) {
+__device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, double *brick) {
     if (cs.x < 0) return;
     const int base = ((cs.z - B.z0) * B.by + (cs.y - B.y0)) * B.bx + (cs.x - B.x0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
-        if (cs.s[k] != 0.0f) atomicAdd(&brick[base + off], cs.s[k]);
+        if (cs.s[k] != 0.0f) atomicAdd(&brick[base + off], (double)cs.s[k]);
     }
     cs.reset();
 }
@@ -165,7 +165,7 @@ __device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, float *b
 // issues no LDS atomic.
 template <bool LDS>
 __device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
-                                           float *brick, double *acc) {
+                                           double *brick, double *acc) {
     if (cs.x >= 0) {
         const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
         const int ax = (dx != 0) + (dy != 0) + (dz != 0) == 1 && abs(dx + dy + dz) == 1
@@ -197,7 +197,11 @@ __device__ __forceinline__ void transition(const Params &P, const Brick &B, Cell
 }
 
 constexpr int kChunk = 32;         // samples per lockstep chunk
-constexpr int kBrickCap = 8192;    // LDS floats per workgroup (32 KiB)
+// FP64 LDS brick entries per workgroup (dynamic LDS, 8 B each).  gfx950
+// executes ds_add_f64 about ten times faster than ds_add_f32 (20 vs 193
+// cycles per wave64 instruction, measured), so the brick accumulates the FP32
+// cell sums in FP64.
+constexpr int kBrickCapDefault = 4096;
 
 // Clamped lower cell coordinate of one FP32 sample (same arithmetic as cell32,
 // without the support test: out-of-support samples only widen the box).
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                                                        const float *__restrict__ seeds,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ partials) {
-    __shared__ float brick[VOL ? kBrickCap : 1];
+    extern __shared__ double brick[];
     __shared__ int bound_red[kBlock / 64][6];
     int view, row, col, wslot;
     bool wave_valid;
@@ -284,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     bool bad = false;
 
     if constexpr (VOL) {
-        for (int i = threadIdx.x; i < kBrickCap; i += kBlock) brick[i] = 0.0f;
+        for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
     }
     const int chunks = (P.samples + kChunk - 1) / kChunk;
     for (int ch = 0; ch < chunks; ++ch) {
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
 #pragma unroll
                 for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
                 const long long vol = (long long)e[0] * e[1] * e[2];
-                if (vol <= kBrickCap) {
+                if (vol <= P.brick_cap) {
                     // every lane holds the same values: keep them in SGPRs
                     use_lds = true;
                     B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     dd0.fma_add(pos, g[0]);
                     dd1.fma_add(pos, g[1]);
                     dd2.fma_add(pos, g[2]);
-                    const float alpha = ((float)k + 0.5f) / (float)P.samples;
+                    const float alpha = sample_alpha(P, k);
                     const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
                     shl.fma_add(1.0f - alpha, along);
                     shu.fma_add(alpha, along);
@@ -434,9 +438,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 // drain the brick: one FP64 global atomic per touched sample
                 const float inv_bx = 1.0f / (float)B.bx, inv_by = 1.0f / (float)B.by;
                 for (int i = threadIdx.x; i < B.volume; i += kBlock) {
-                    const float v = brick[i];
-                    if (v != 0.0f) {
-                        brick[i] = 0.0f;
+                    const double v = brick[i];
+                    if (v != 0.0) {
+                        brick[i] = 0.0;
                         int q = (int)((float)i * inv_bx);
                         q -= (q * B.bx > i) ? 1 : 0;
                         q += ((q + 1) * B.bx <= i) ? 1 : 0;
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                         z += ((z + 1) * B.by <= q) ? 1 : 0;
                         const int y = q - z * B.by;
                         const int g = ((B.z0 + z) * P.ny + (B.y0 + y)) * P.nx + (B.x0 + x);
-                        unsafeAtomicAdd(&acc[g], (double)v);
+                        unsafeAtomicAdd(&acc[g], v);
                     }
                 }
             } else {
@@ -920,6 +924,7 @@ struct dpt_plan {
     int32_t *h_status = nullptr;
     double *d_partials = nullptr;
     double *d_acc = nullptr;
+    float *d_alpha = nullptr;
     unsigned long long *d_count = nullptr;
     int vjp_vb = 1;          // views per workgroup of the VJP launches
     int forward_vb = 1;      // views per workgroup of the forward launch
@@ -1050,11 +1055,12 @@ template <typename T, int C, bool PAIR>
 void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, hipStream_t s) {
     constexpr bool HALF = !std::is_same<T, float>::value;
     const dim3 grid(tile_grid(P)), block(kBlock);
+    const size_t lds = (size_t)P.brick_cap * sizeof(double);
     if (vol && pose)
-        hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
+        hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, PAIR>), grid, block, lds, s, P, f, sd, p->d_acc,
                            p->d_partials);
     else if (vol)
-        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
+        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, PAIR>), grid, block, lds, s, P, f, sd, p->d_acc,
                            p->d_partials);
     else
         hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
@@ -1122,6 +1128,7 @@ int dpt_plan_destroy(dpt_plan *p) {
     (void)hipFree(p->d_status);
     (void)hipFree(p->d_partials);
     (void)hipFree(p->d_acc);
+    (void)hipFree(p->d_alpha);
     (void)hipFree(p->d_count);
     if (p->h_status) (void)hipHostFree(p->h_status);
     for (auto &e : p->ev)
@@ -1187,6 +1194,11 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     P.samples = samples;
     P.bundle = panels == nullptr ? 1 : 0;
     configure_tiles(P, 1);
+    P.brick_cap = kBrickCapDefault;
+    if (const char *cap = std::getenv("DPT_AMD_BRICK_CAP")) {
+        const int forced = std::atoi(cap);
+        if (forced >= 512 && forced <= 8192) P.brick_cap = forced;
+    }
     p->vjp_vb = P.bundle ? 1 : choose_views_per_block(panels, views, rows, cols);
     if (const char *vb = std::getenv("DPT_AMD_VIEWS_PER_BLOCK")) {
         const int forced = std::atoi(vb);
@@ -1231,6 +1243,14 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * views);
     if (needs & DPT_VOLUME) ALLOC(p->d_acc, sizeof(double) * (size_t)voxels);
     ALLOC(p->d_count, sizeof(unsigned long long));
+    ALLOC(p->d_alpha, sizeof(float) * (size_t)samples);
+    {
+        std::vector<float> alpha((size_t)samples);
+        fill_alpha_table(alpha.data(), samples);
+        if (hipMemcpy(p->d_alpha, alpha.data(), sizeof(float) * (size_t)samples, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(DPT_E_DEVICE, "copy of the sample fractions failed");
+    }
+    P.alpha = p->d_alpha;
     for (auto &e : p->ev)
         if (hipEventCreate(&e) != hipSuccess) return cleanup(DPT_E_DEVICE, "event creation failed");
 #undef ALLOC
