@@ -208,6 +208,8 @@ class _Active:
     state: object
     done: int = 0
     admitted: float = 0.0
+    staging: tuple = ()  # pinned host sources of the asynchronous uploads
+    serial: int = 0      # unique per admission (descriptor cache key)
 
 
 @dataclass
@@ -273,6 +275,8 @@ class PoseFitEngine:
         self._handle = handle
         # graph capture needs a non-default stream (default priority)
         self.stream = torch.cuda.Stream(device=self.device)
+        self._desc_cache = ((), None)
+        self._serial = 0
         self._lock = threading.Lock()
 
     def close(self) -> None:
@@ -292,14 +296,24 @@ class PoseFitEngine:
         if request.volume not in self.registry:
             raise ContractError(f"request {request.key}: volume {request.volume!r} is not resident")
         v, r, c = request.geometry.shape
-        targets = torch.as_tensor(np.ascontiguousarray(request.targets, dtype=np.float32)).reshape(v, r, c)
-        targets = targets.to(self.device, non_blocking=False)
-        panels = torch.tensor([p.packed() for p in request.geometry.views], dtype=torch.float64, device=self.device)
-        state = torch.zeros(STATE_SIZE, dtype=torch.float64, device=self.device)
-        state[POSE : POSE + 12] = torch.tensor(request.initial_pose, dtype=torch.float64, device=self.device)
-        return _Active(request, targets, panels, state, 0, now)
+        # pinned host copies uploaded asynchronously on the engine stream: an
+        # admission does not wait for the steps already in flight
+        host_targets = torch.as_tensor(np.ascontiguousarray(request.targets, dtype=np.float32)).reshape(v, r, c)
+        host_targets = host_targets.pin_memory()
+        host_panels = torch.tensor([p.packed() for p in request.geometry.views], dtype=torch.float64).pin_memory()
+        host_state = torch.zeros(STATE_SIZE, dtype=torch.float64)
+        host_state[POSE : POSE + 12] = torch.tensor(request.initial_pose, dtype=torch.float64)
+        host_state = host_state.pin_memory()
+        targets = host_targets.to(self.device, non_blocking=True)
+        panels = host_panels.to(self.device, non_blocking=True)
+        state = host_state.to(self.device, non_blocking=True)
+        self._serial += 1
+        return _Active(request, targets, panels, state, 0, now, (host_targets, host_panels, host_state), self._serial)
 
     def _descs(self, active: list[_Active]):
+        signature = tuple(a.serial for a in active)
+        if signature == self._desc_cache[0]:
+            return self._desc_cache[1]
         keys = sorted({a.request.volume for a in active})
         index = {k: i for i, k in enumerate(keys)}
         vols = (VolumeDesc * len(keys))(*[self.registry[k].desc() for k in keys])
@@ -312,6 +326,7 @@ class PoseFitEngine:
             q.panels = a.panels.data_ptr()
             q.targets = a.targets.data_ptr()
             q.state = a.state.data_ptr()
+        self._desc_cache = (signature, (vols, len(keys), reqs))
         return vols, len(keys), reqs
 
     def step(self, active: list[_Active]) -> None:
@@ -343,9 +358,30 @@ class PoseFitEngine:
         used = 0
         steps = peak = peak_bytes = 0
         inflight: deque = deque()
+        completions: deque = deque()
         start = time.perf_counter()
+
+        def emit(landed, snapshot, finished):
+            stop = time.perf_counter() - start
+            states = snapshot.numpy()
+            for a, st in zip(finished, states):
+                results.append(
+                    FitResult(
+                        a.request.key,
+                        st[BEST_POSE : BEST_POSE + 12].copy(),
+                        float(st[BEST_LOSS]),
+                        int(st[EVALS]),
+                        int(st[ACCEPTED]),
+                        a.request.arrival if realtime else 0.0,
+                        a.admitted,
+                        stop,
+                    )
+                )
+
         with self._lock, torch.cuda.stream(self.stream):
             while pending or queued or active:
+                while completions and completions[0][0].query():
+                    emit(*completions.popleft())
                 now = time.perf_counter() - start
                 while pending and (not realtime or pending[0].arrival <= now):
                     queued.append(pending.popleft())
@@ -372,34 +408,29 @@ class PoseFitEngine:
                 for a in active:
                     a.done += 1
                 finished = [a for a in active if a.done >= a.request.iterations]
-                # Host and device overlap: synchronise only when a solve ends;
-                # otherwise keep at most two steps in flight (the native table
-                # staging is double-buffered).
+                if finished:
+                    # The final states are copied to pinned memory behind the
+                    # step on the same stream; the solves leave the batch now
+                    # and their results are emitted when the copy has landed,
+                    # without draining the steps in flight.
+                    snapshot = torch.empty((len(finished), STATE_SIZE), dtype=torch.float64, pin_memory=True)
+                    snapshot.copy_(torch.stack([a.state for a in finished]), non_blocking=True)
+                    landed = torch.cuda.Event()
+                    landed.record(self.stream)
+                    completions.append((landed, snapshot, finished))
+                    for a in finished:
+                        used -= a.request.nbytes
+                    active = [a for a in active if a.done < a.request.iterations]
+                # keep at most two steps in flight (the native table staging is
+                # double-buffered)
                 event = torch.cuda.Event()
                 event.record(self.stream)
                 inflight.append(event)
-                if not finished and len(inflight) > 2:
+                if len(inflight) > 2:
                     inflight.popleft().synchronize()
-                if finished:
-                    self.stream.synchronize()
-                    inflight.clear()
-                    stop = time.perf_counter() - start
-                    states = torch.stack([a.state for a in finished]).cpu().numpy()
-                    for a, st in zip(finished, states):
-                        results.append(
-                            FitResult(
-                                a.request.key,
-                                st[BEST_POSE : BEST_POSE + 12].copy(),
-                                float(st[BEST_LOSS]),
-                                int(st[EVALS]),
-                                int(st[ACCEPTED]),
-                                a.request.arrival if realtime else 0.0,
-                                a.admitted,
-                                stop,
-                            )
-                        )
-                        used -= a.request.nbytes
-                    active = [a for a in active if a.done < a.request.iterations]
+            while completions:
+                completions[0][0].synchronize()
+                emit(*completions.popleft())
         return RunStats(results, time.perf_counter() - start, steps, peak, peak_bytes)
 
 
