@@ -103,6 +103,15 @@ def test_host_and_torch_compose_pose_agree_and_act_on_the_right():
     assert moved == pytest.approx((1.0, 3.0, 3.0))
 
 
+def test_torch_pose_chart_passes_gradcheck():
+    """The exp-map chart's autograd (matrix_exp) against finite differences,
+    as the reference pins it (tests/python/test_projector_workflow.py:96-98)."""
+    anchor = torch.tensor(RigidTransform((0.0, -1.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0), (1.0, 2.0, 3.0)).packed(),
+                          dtype=torch.float64)
+    xi = torch.tensor([0.3, -0.2, 0.1, 0.05, -0.07, 0.02], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda x: torch_compose(anchor, x), (xi,), eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
 def test_grid_object_to_grid_matrix_is_diag_inverse_spacing_times_orientation_transpose():
     c, s = math.cos(0.3), math.sin(0.3)
     grid = GridSpec((2, 3, 4), (0.5, 2.0, 4.0), (1.0, 2.0, 3.0), (c, -s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0))
