@@ -41,15 +41,23 @@ struct Params {
     const double *poses;     // views x 12
     int *status;             // views
     int brick_cap;           // FP64 LDS brick entries per workgroup (volume adjoint)
-    const float *alpha;      // samples entries: RN((k + 0.5) / samples), FP32
+    const float *alpha;      // 2 x samples entries: alpha_k, then k + 0.5 (sample_table_entries)
 };
 
-// alpha_k = RN((k + 0.5) / S) of the pose clip-face shifts
-// (kernels/projection_fp32.py:455-458), tabulated once on the host (IEEE
-// binary32 division, identical to the device's correctly rounded division):
-// the per-sample division is replaced by one load of a uniform address.
+// Per-sample constants, tabulated once on the host so that the sample loop
+// reads them with scalar loads of a uniform address instead of computing
+// them per lane:
+//   alpha_k = RN((k + 0.5) / S) of the pose clip-face shifts
+//     (kernels/projection_fp32.py:455-458; IEEE binary32 division, identical
+//     to the device's correctly rounded division);
+//   c_k = k + 0.5, the sample centre (kernels/projection_fp32.py:116-119;
+//     exact in FP32 for k < 2^23).
+inline size_t sample_table_entries(int samples) { return 2 * (size_t)samples; }
 inline void fill_alpha_table(float *out, int samples) {
-    for (int k = 0; k < samples; ++k) out[k] = ((float)k + 0.5f) / (float)samples;
+    for (int k = 0; k < samples; ++k) {
+        out[k] = ((float)k + 0.5f) / (float)samples;
+        out[samples + k] = (float)k + 0.5f;
+    }
 }
 
 // k is uniform across the wave: read the table through the constant address
@@ -57,6 +65,10 @@ inline void fill_alpha_table(float *out, int samples) {
 __device__ __forceinline__ float sample_alpha(const Params &P, int k) {
     typedef const float __attribute__((address_space(4))) *constant_t;
     return ((constant_t)P.alpha)[k];
+}
+__device__ __forceinline__ float sample_centre(const Params &P, int k) {
+    typedef const float __attribute__((address_space(4))) *constant_t;
+    return ((constant_t)P.alpha)[P.samples + k];
 }
 
 // ------------------------------------------------------------------ utils --
@@ -258,15 +270,13 @@ __device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, floa
     const float v0 = __builtin_amdgcn_fmed3f(p0, 0.0f, P.cx);
     const float v1 = __builtin_amdgcn_fmed3f(p1, 0.0f, P.cy);
     const float v2 = __builtin_amdgcn_fmed3f(p2, 0.0f, P.cz);
-    const float f0 = fminf(floorf(v0), (float)P.capx);
-    const float f1 = fminf(floorf(v1), (float)P.capy);
-    const float f2 = fminf(floorf(v2), (float)P.capz);
-    c.lx = (int)f0;
-    c.ly = (int)f1;
-    c.lz = (int)f2;
-    c.fx = v0 - f0;
-    c.fy = v1 - f1;
-    c.fz = v2 - f2;
+    // v >= 0, so truncation is floor; v - lower is exact (Sterbenz)
+    c.lx = min((int)v0, P.capx);
+    c.ly = min((int)v1, P.capy);
+    c.lz = min((int)v2, P.capz);
+    c.fx = v0 - (float)c.lx;
+    c.fy = v1 - (float)c.ly;
+    c.fz = v2 - (float)c.lz;
     c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
     c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
     c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
@@ -445,7 +455,7 @@ struct Acc<true> {
 // support (kernels/projection_fp32.py:116-127, 130-140).
 __device__ __forceinline__ void sample_at(const Params &P, const float *o, const float *d, float lower, float step,
                                           int k, float &pos, Cell32 &c) {
-    pos = fmaf((float)k + 0.5f, step, lower);
+    pos = fmaf(sample_centre(P, k), step, lower);
     const float p0 = fmaf(pos, d[0], o[0]);
     const float p1 = fmaf(pos, d[1], o[1]);
     const float p2 = fmaf(pos, d[2], o[2]);

@@ -494,7 +494,7 @@ int dpt_fit_step(dpt_fit *f, const dpt_volume_desc *volumes, int32_t n_volumes, 
         }
         auto table = f->alpha.find(q.samples);
         if (table == f->alpha.end()) {
-            std::vector<float> host((size_t)q.samples);
+            std::vector<float> host(sample_table_entries(q.samples));
             fill_alpha_table(host.data(), q.samples);
             float *dev = nullptr;
             if (hipMalloc(&dev, sizeof(float) * host.size()) != hipSuccess ||

@@ -126,16 +126,6 @@ struct CellSums {
     }
 };
 
-// Flush to the FP64 global accumulator.
-__device__ __forceinline__ void flush_global(const Params &P, CellSums &cs, double *acc) {
-    if (cs.x < 0) return;
-    const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (cs.s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.s[k]);
-    cs.reset();
-}
-
 // Brick = axis-aligned box of grid samples that one workgroup's rays touch
 // during one chunk of samples, privatised in LDS.
 struct Brick {
@@ -145,14 +135,30 @@ struct Brick {
     int volume;
 };
 
-__device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, double *brick) {
-    if (cs.x < 0) return;
-    const int base = ((cs.z - B.z0) * B.by + (cs.y - B.y0)) * B.bx + (cs.x - B.x0);
+// Add the non-zero values f[k] at the eight corners of the register cell:
+// into the LDS brick (FP64 LDS atomics) or into the FP64 global accumulator.
+template <bool LDS>
+__device__ __forceinline__ void add_corners(const Params &P, const Brick &B, const CellSums &cs, const float *f,
+                                            double *brick, double *acc) {
+    if constexpr (LDS) {
+        const int base = ((cs.z - B.z0) * B.by + (cs.y - B.y0)) * B.bx + (cs.x - B.x0);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
-        if (cs.s[k] != 0.0f) atomicAdd(&brick[base + off], (double)cs.s[k]);
+        for (int k = 0; k < 8; ++k) {
+            const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
+            if (f[k] != 0.0f) atomicAdd(&brick[base + off], (double)f[k]);
+        }
+    } else {
+        const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (f[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)f[k]);
     }
+}
+
+template <bool LDS>
+__device__ __forceinline__ void flush(const Params &P, const Brick &B, CellSums &cs, double *brick, double *acc) {
+    if (cs.x < 0) return;
+    add_corners<LDS>(P, B, cs, cs.s, brick, acc);
     cs.reset();
 }
 
@@ -160,37 +166,43 @@ __device__ __forceinline__ void flush_lds(const Brick &B, CellSums &cs, double *
 // (kernels/projection_fp32.py:272-331): when the new cell is the old one
 // stepped by +-1 along one axis, the four corners on the shared face stay in
 // registers (renamed to the opposite side of the new cell) and only the four
-// departing corners are flushed; any other move flushes all eight.  Selects
-// instead of branches keep the wave converged; a flushed lane with a zero sum
-// issues no LDS atomic.
+// departing corners are flushed; any other move flushes all eight.
+//
+// Inside a lockstep sample loop some lane of a wave changes cell at nearly
+// every sample, so this code runs at wave level once per sample: it is
+// written as lane masks and selects (no data-dependent branches besides the
+// per-corner atomic masks).  An empty register cell (x < 0) holds zeros, so
+// it flushes nothing and carries zeros.
 template <bool LDS>
 __device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
                                            double *brick, double *acc) {
-    if (cs.x >= 0) {
-        const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
-        const int ax = (dx != 0) + (dy != 0) + (dz != 0) == 1 && abs(dx + dy + dz) == 1
-                           ? (dx != 0 ? 0 : (dy != 0 ? 1 : 2))
-                           : -1;
-        const int bit = ax < 0 ? 0 : (1 << ax);
-        const int keep = (dx + dy + dz) > 0 ? bit : 0;  // corners with this bit value are shared
-        float carried[8];
+    const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
+    const bool live = cs.x >= 0;
+    const bool on_x = live && dy == 0 && dz == 0, on_y = live && dx == 0 && dz == 0,
+               on_z = live && dx == 0 && dy == 0;
+    // +1 / -1 unit steps along each axis; at most one is set
+    const bool px = on_x && dx == 1, mx = on_x && dx == -1;
+    const bool py = on_y && dy == 1, my = on_y && dy == -1;
+    const bool pz = on_z && dz == 1, mz = on_z && dz == -1;
+    float departing[8], carried[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const bool shared = ax >= 0 && (k & bit) == keep;
-            carried[k] = shared ? cs.s[k] : 0.0f;
-            if (shared) cs.s[k] = 0.0f;
-        }
-        if constexpr (LDS)
-            flush_lds(B, cs, brick);
-        else
-            flush_global(P, cs, acc);
-        // renamed: corner k of the old cell is corner k ^ bit of the new one
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float from1 = carried[j ^ 1], from2 = carried[j ^ 2], from4 = carried[j ^ 4];
-            cs.s[j] = bit == 1 ? from1 : (bit == 2 ? from2 : (bit == 4 ? from4 : 0.0f));
-        }
+    for (int k = 0; k < 8; ++k) {
+        // old corner k lies on the shared face: its bit along the step axis is
+        // set for a +1 step and clear for a -1 step
+        const bool shared = ((k & 1) ? px : mx) || ((k & 2) ? py : my) || ((k & 4) ? pz : mz);
+        departing[k] = shared ? 0.0f : cs.s[k];
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        // new corner j is old corner j ^ bit when that corner is shared
+        carried[j] = ((j & 1) ? mx : px)   ? cs.s[j ^ 1]
+                     : ((j & 2) ? my : py) ? cs.s[j ^ 2]
+                     : ((j & 4) ? mz : pz) ? cs.s[j ^ 4]
+                                           : 0.0f;
+    }
+    add_corners<LDS>(P, B, cs, departing, brick, acc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cs.s[j] = carried[j];
     cs.x = c.lx;
     cs.y = c.ly;
     cs.z = c.lz;
@@ -367,9 +379,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 if (c.base < 0) {  // outside support: nothing to add, close the cell
                     if constexpr (VOL) {
                         if (use_lds)
-                            flush_lds(B, cs, brick);
+                            flush<true>(P, B, cs, brick, acc);
                         else
-                            flush_global(P, cs, acc);
+                            flush<false>(P, B, cs, brick, acc);
                     }
                     cur = -1;
                     continue;
@@ -407,8 +419,21 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 }
                 if constexpr (VOL) {
                     if (bounded) {
+                        // The flush-to-zero rule of scatter_increment changes
+                        // an increment only when its magnitude is below
+                        // 2^-126: apply it when some lane of the wave has one.
+                        float inc[8];
 #pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += scatter_increment(volume_factor, wt[kk]);
+                        for (int kk = 0; kk < 8; ++kk) inc[kk] = volume_factor * wt[kk];
+                        const float smallest =
+                            fminf(fminf(fminf(fabsf(inc[0]), fabsf(inc[1])), fminf(fabsf(inc[2]), fabsf(inc[3]))),
+                                  fminf(fminf(fabsf(inc[4]), fabsf(inc[5])), fminf(fabsf(inc[6]), fabsf(inc[7]))));
+                        if (__builtin_amdgcn_ballot_w64(smallest < 0x1p-126f) != 0) {
+#pragma unroll
+                            for (int kk = 0; kk < 8; ++kk) inc[kk] = scatter_increment(volume_factor, wt[kk]);
+                        }
+#pragma unroll
+                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += inc[kk];
                     } else {
 #pragma unroll
                         for (int kk = 0; kk < 8; ++kk) {
@@ -433,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         }
         if constexpr (VOL) {
             if (use_lds) {
-                flush_lds(B, cs, brick);
+                flush<true>(P, B, cs, brick, acc);
                 __syncthreads();
                 // drain the brick: one FP64 global atomic per touched sample
                 const float inv_bx = 1.0f / (float)B.bx, inv_by = 1.0f / (float)B.by;
@@ -454,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     }
                 }
             } else {
-                flush_global(P, cs, acc);
+                flush<false>(P, B, cs, brick, acc);
             }
             __syncthreads();
         }
@@ -1243,11 +1268,11 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * views);
     if (needs & DPT_VOLUME) ALLOC(p->d_acc, sizeof(double) * (size_t)voxels);
     ALLOC(p->d_count, sizeof(unsigned long long));
-    ALLOC(p->d_alpha, sizeof(float) * (size_t)samples);
+    ALLOC(p->d_alpha, sizeof(float) * sample_table_entries(samples));
     {
-        std::vector<float> alpha((size_t)samples);
+        std::vector<float> alpha(sample_table_entries(samples));
         fill_alpha_table(alpha.data(), samples);
-        if (hipMemcpy(p->d_alpha, alpha.data(), sizeof(float) * (size_t)samples, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(p->d_alpha, alpha.data(), sizeof(float) * alpha.size(), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(DPT_E_DEVICE, "copy of the sample fractions failed");
     }
     P.alpha = p->d_alpha;
