@@ -260,27 +260,42 @@ struct Cell32 {
     float slx, sly, slz;     // slopes (0/1)
 };
 
-// kernels/projection_fp32.py:64-74 + 130-140. Returns false outside support
-// (NaN and infinities fail the comparisons, as finite_point does).  For the
-// finite coordinates that reach the clamp, med3 equals min(max(p, 0), n - 1)
-// and min(floor(v), n - 2) in float equals the reference's integer min.
-__device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, float p2, Cell32 &c) {
-    if (!(p0 >= -0.5f && p1 >= -0.5f && p2 >= -0.5f && p0 <= P.hx && p1 <= P.hy && p2 <= P.hz))
-        return false;
+// Maximum ny * nz of a grid: cell indices use 24-bit multiplies
+// ((lz * ny + ly) * nx + lx with both products' operands below 2^24).
+constexpr long long kMaxSliceProduct = 1LL << 24;
+
+// Support test of one FP32 sample point (kernels/projection_fp32.py:64-74):
+// NaN and infinities fail the comparisons, as finite_point does.
+__device__ __forceinline__ bool inside32(const Params &P, float p0, float p1, float p2) {
+    return p0 >= -0.5f && p1 >= -0.5f && p2 >= -0.5f && p0 <= P.hx && p1 <= P.hy && p2 <= P.hz;
+}
+
+// kernels/projection_fp32.py:130-140 for a point inside the support.  For
+// the finite coordinates that reach the clamp, med3 equals
+// min(max(p, 0), n - 1), and min(trunc(v), n - 2) equals the reference's
+// integer min of floor(v) (v >= 0).
+__device__ __forceinline__ void cell32_inside(const Params &P, float p0, float p1, float p2, Cell32 &c) {
     const float v0 = __builtin_amdgcn_fmed3f(p0, 0.0f, P.cx);
     const float v1 = __builtin_amdgcn_fmed3f(p1, 0.0f, P.cy);
     const float v2 = __builtin_amdgcn_fmed3f(p2, 0.0f, P.cz);
-    // v >= 0, so truncation is floor; v - lower is exact (Sterbenz)
     c.lx = min((int)v0, P.capx);
     c.ly = min((int)v1, P.capy);
     c.lz = min((int)v2, P.capz);
+    // v - lower is exact (Sterbenz)
     c.fx = v0 - (float)c.lx;
     c.fy = v1 - (float)c.ly;
     c.fz = v2 - (float)c.lz;
     c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
     c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
     c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
-    c.base = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+    c.base = (int)(__umul24(__umul24((unsigned)c.lz, (unsigned)P.ny) + (unsigned)c.ly, (unsigned)P.nx) +
+                   (unsigned)c.lx);
+}
+
+// Returns false outside the support.
+__device__ __forceinline__ bool cell32(const Params &P, float p0, float p1, float p2, Cell32 &c) {
+    if (!inside32(P, p0, p1, p2)) return false;
+    cell32_inside(P, p0, p1, p2, c);
     return true;
 }
 
@@ -452,14 +467,31 @@ struct Acc<true> {
 };
 
 // Position of sample k and its interpolation cell; c.base = -1 outside the
-// support (kernels/projection_fp32.py:116-127, 130-140).
+// support (kernels/projection_fp32.py:116-127, 130-140).  CHECK = false skips
+// the support test for a span of samples that span_inside has cleared.
+template <bool CHECK = true>
 __device__ __forceinline__ void sample_at(const Params &P, const float *o, const float *d, float lower, float step,
                                           int k, float &pos, Cell32 &c) {
     pos = fmaf(sample_centre(P, k), step, lower);
     const float p0 = fmaf(pos, d[0], o[0]);
     const float p1 = fmaf(pos, d[1], o[1]);
     const float p2 = fmaf(pos, d[2], o[2]);
-    if (!cell32(P, p0, p1, p2, c)) c.base = -1;
+    if constexpr (CHECK) {
+        if (!cell32(P, p0, p1, p2, c)) c.base = -1;
+    } else {
+        cell32_inside(P, p0, p1, p2, c);
+    }
+}
+
+// Every sample point of [k0, k1) lies in the support iff the first and the
+// last do: pos_k = RN(c_k * step + lower) is non-decreasing in k (step > 0)
+// and each coordinate RN(pos * d_a + o_a) is monotone in pos, so the samples
+// inside the (convex) support along each axis form a run of consecutive k.
+__device__ __forceinline__ bool span_inside(const Params &P, const float *o, const float *d, float lower, float step,
+                                            int k0, int k1) {
+    const float a = fmaf(sample_centre(P, k0), step, lower), b = fmaf(sample_centre(P, k1 - 1), step, lower);
+    return inside32(P, fmaf(a, d[0], o[0]), fmaf(a, d[1], o[1]), fmaf(a, d[2], o[2])) &&
+           inside32(P, fmaf(b, d[0], o[0]), fmaf(b, d[1], o[1]), fmaf(b, d[2], o[2]));
 }
 
 // The reference flags any non-finite sample point (projection_fp32.py:366-367).

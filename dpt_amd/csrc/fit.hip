@@ -88,6 +88,9 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
     dd2.init();
     float v[8];
     int cur = -1;
+    // per-sample support tests only for waves with a ray whose end samples
+    // leave the support (uniform flag)
+    const bool checked = __builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, 0, P.samples)) != 0;
     for (int k = 0; k < P.samples; ++k) {
         if constexpr (COMPENSATED) {
             if (k > 0 && (k % kFoldBlock) == 0) {
@@ -104,8 +107,12 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
         }
         float pos;
         Cell32 c;
-        sample_at(P, o, d, iv.lower, step, k, pos, c);
-        if (c.base < 0) continue;
+        if (checked) {
+            sample_at<true>(P, o, d, iv.lower, step, k, pos, c);
+            if (c.base < 0) continue;
+        } else {
+            sample_at<false>(P, o, d, iv.lower, step, k, pos, c);
+        }
         if (c.base != cur) {
             if (P.sx)
                 load8<T, true>(P, field, c.base, v);
@@ -369,6 +376,10 @@ int fit_fail(int code, const std::string &msg) {
 bool fill_params(Params &P, const dpt_volume_desc &v, const dpt_fit_request &q, std::string &why) {
     if (v.nx < 1 || v.ny < 1 || v.nz < 1 || (int64_t)v.nx * v.ny * v.nz > INT32_MAX) {
         why = "volume counts must be positive and fit signed 32-bit indexing";
+        return false;
+    }
+    if ((long long)v.ny * v.nz > kMaxSliceProduct) {
+        why = "volume ny * nz exceeds 2^24 (24-bit cell index arithmetic)";
         return false;
     }
     if (q.views < 1 || q.rows < 1 || q.cols < 1 || q.samples < 1) {

@@ -77,22 +77,33 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
         bad = !finite_span(o, d, iv.lower, step, 0, P.samples);
         float v[8];
         int cur = -1;
-        for (int k = 0; k < P.samples; ++k) {
-            if constexpr (COMPENSATED) {
-                if (k > 0 && (k % kFoldBlock) == 0) total.fold();
+        auto walk = [&](auto check) {
+            constexpr bool CHECK = decltype(check)::value;
+            for (int k = 0; k < P.samples; ++k) {
+                if constexpr (COMPENSATED) {
+                    if (k > 0 && (k % kFoldBlock) == 0) total.fold();
+                }
+                float pos;
+                Cell32 c;
+                sample_at<CHECK>(P, o, d, iv.lower, step, k, pos, c);
+                if constexpr (CHECK) {
+                    if (c.base < 0) continue;
+                }
+                if (c.base != cur) {
+                    load8<T, PAIR>(P, field, c.base, v);
+                    cur = c.base;
+                }
+                float wt[8];
+                weights32(c, wt);
+                total.add(interpolate32(wt, v));
             }
-            float pos;
-            Cell32 c;
-            sample_at(P, o, d, iv.lower, step, k, pos, c);
-            if (c.base < 0) continue;
-            if (c.base != cur) {
-                load8<T, PAIR>(P, field, c.base, v);
-                cur = c.base;
-            }
-            float wt[8];
-            weights32(c, wt);
-            total.add(interpolate32(wt, v));
-        }
+        };
+        // per-sample support tests only for waves with a ray whose end
+        // samples leave the support
+        if (__builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, 0, P.samples)) == 0)
+            walk(std::false_type{});
+        else
+            walk(std::true_type{});
         if (bad) flag(P, view, DPT_STATUS_RANGE);
         result = total.value() * (len * step);
     }
@@ -141,7 +152,9 @@ template <bool LDS>
 __device__ __forceinline__ void add_corners(const Params &P, const Brick &B, const CellSums &cs, const float *f,
                                             double *brick, double *acc) {
     if constexpr (LDS) {
-        const int base = ((cs.z - B.z0) * B.by + (cs.y - B.y0)) * B.bx + (cs.x - B.x0);
+        const int base = (int)(__umul24(__umul24((unsigned)(cs.z - B.z0), (unsigned)B.by) + (unsigned)(cs.y - B.y0),
+                                        (unsigned)B.bx) +
+                               (unsigned)(cs.x - B.x0));
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
@@ -372,19 +385,29 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             bad |= !finite_span(o, d, iv.lower, step, k0, k1);
             float v[8];
             int cur = -1;
+            // Per-sample support tests only for waves with a ray whose chunk
+            // ends leave the support.  Both choices are uniform over the wave
+            // and stay runtime flags: two specialised copies of this loop
+            // raise the register allocation past four waves per SIMD.
+            const bool checked = __builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, k0, k1)) != 0;
             for (int k = k0; k < k1; ++k) {
                 float pos;
                 Cell32 c;
-                sample_at(P, o, d, iv.lower, step, k, pos, c);
-                if (c.base < 0) {  // outside support: nothing to add, close the cell
-                    if constexpr (VOL) {
-                        if (use_lds)
-                            flush<true>(P, B, cs, brick, acc);
-                        else
-                            flush<false>(P, B, cs, brick, acc);
+                if (checked)
+                    sample_at<true>(P, o, d, iv.lower, step, k, pos, c);
+                else
+                    sample_at<false>(P, o, d, iv.lower, step, k, pos, c);
+                if (checked) {
+                    if (c.base < 0) {  // outside support: nothing to add, close the cell
+                        if constexpr (VOL) {
+                            if (use_lds)
+                                flush<true>(P, B, cs, brick, acc);
+                            else
+                                flush<false>(P, B, cs, brick, acc);
+                        }
+                        cur = -1;
+                        continue;
                     }
-                    cur = -1;
-                    continue;
                 }
                 if (c.base != cur) {
                     if constexpr (POSE) load8<T, PAIR>(P, field, c.base, v);
@@ -1171,6 +1194,8 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         return fail(DPT_E_INVALID, "grid counts must be positive");
     const int64_t voxels = (int64_t)grid->nx * grid->ny * grid->nz;
     if (voxels > INT32_MAX) return fail(DPT_E_INVALID, "grid exceeds signed 32-bit indexing");
+    if ((long long)grid->ny * grid->nz > kMaxSliceProduct)
+        return fail(DPT_E_INVALID, "grid ny * nz exceeds 2^24 (24-bit cell index arithmetic)");
     if (views < 1 || rows < 1 || cols < 1) return fail(DPT_E_INVALID, "views, rows and cols must be positive");
     const int64_t pixels = (int64_t)views * rows * cols;
     if (pixels > INT32_MAX) return fail(DPT_E_INVALID, "image exceeds signed 32-bit indexing");
