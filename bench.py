@@ -181,11 +181,13 @@ def serving_benchmark(count: int, rate: float) -> dict:
     arrivals = serving.poisson_arrivals(count, rate, seed=405)
     timed = [dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, arrivals)]
     out = {"requests": count, "arrival_rate_per_s": rate}
-    forms = {"batched_bf16": ("bfloat16", 64, True), "plain_fp32": ("float32", 1, False)}
-    for name, (storage, slots, graph) in forms.items():
+    # the served form keeps volumes resident in BF16 cell layout; the plain
+    # form gathers FP32 from the linear array, as the reference does
+    forms = {"batched_bf16": ("bfloat16", "cells", 64, True), "plain_fp32": ("float32", "linear", 1, False)}
+    for name, (storage, layout, slots, graph) in forms.items():
         registry = serving.VolumeRegistry()
         for key, grid, values in volumes:
-            registry.add(key, values, grid, storage=storage)
+            registry.add(key, values, grid, storage=storage, layout=layout)
         engine = serving.PoseFitEngine(registry, max_requests=slots, max_tiles=1 << 19, use_graph=graph)
         engine.run(requests[:2], realtime=False)  # warm-up: code objects, graphs
         closed = engine.run(requests, realtime=False)
@@ -197,6 +199,7 @@ def serving_benchmark(count: int, rate: float) -> dict:
         errors = np.asarray(errors)
         out[name] = {
             "storage": storage,
+            "layout": layout,
             "slots": slots,
             "graph": graph,
             "closed_loop": closed.summary(),
@@ -446,7 +449,7 @@ def main() -> int:
     import torch
 
     from dpt_amd import phantoms
-    from dpt_amd.projection import prepare
+    from dpt_amd.projection import decode, gather_cells, prepare
     from dpt_amd.torch import Projector
 
     device = torch.device("cuda:0")
@@ -549,6 +552,7 @@ def main() -> int:
             "nominal_samples_per_pass": nominal,
             "effective_samples_per_pass": effective,
             "storage": args.storage,
+            "gather": "cells" if gather_cells(decode(make()._configuration)) else "linear",
         },
         "roofline": {
             "bound": "hbm",

@@ -25,6 +25,8 @@ EXPORTS = (
     "dpt_vjp",
     "dpt_status",
     "dpt_convert_field",
+    "dpt_cell_count",
+    "dpt_convert_cells",
     "dpt_set_timing",
     "dpt_kernel_ms",
     "dpt_count_hit_rays",
@@ -44,7 +46,11 @@ EXPORTS = (
 
 ARITH_FP32, ARITH_FP64 = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
+# OR-ed into a storage code: the field is in the cell layout (dpt_convert_cells)
+STORE_CELLS = 8
 VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
+# plan option (needs): FP32 sampling gathers from a cell-layout copy
+GATHER_CELLS = 64
 
 STORAGE_CODES = {"float32": STORE_F32, "bfloat16": STORE_BF16, "float16": STORE_F16}
 # 16-bit field buffers carry DPT_HALF_PAD_BYTES (8) of readable padding
@@ -172,6 +178,10 @@ def lib():
         h.dpt_status.restype = ctypes.c_int
         h.dpt_convert_field.argtypes = [vp, vp, i64, i32, vp, vp]
         h.dpt_convert_field.restype = ctypes.c_int
+        h.dpt_cell_count.argtypes = [i32, i32, i32]
+        h.dpt_cell_count.restype = i64
+        h.dpt_convert_cells.argtypes = [vp, vp, i32, i32, i32, i32, vp]
+        h.dpt_convert_cells.restype = ctypes.c_int
         h.dpt_set_timing.argtypes = [vp, i32]
         h.dpt_set_timing.restype = ctypes.c_int
         h.dpt_kernel_ms.argtypes = [vp, i32]
@@ -294,6 +304,21 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def cell_count(shape) -> int:
+    """Cells of a (nz, ny, nx) grid in the cell layout: prod(max(n - 1, 1))."""
+    nz, ny, nx = (int(n) for n in shape)
+    count = lib().dpt_cell_count(nx, ny, nz)
+    if count < 0:
+        check(int(count), "dpt_cell_count")
+    return int(count)
+
+
+def convert_cells(src: int, dst: int, shape, storage: str, stream: int) -> None:
+    """Rewrite a linear (nz, ny, nx) field into the cell layout (same dtype)."""
+    nz, ny, nx = (int(n) for n in shape)
+    check(lib().dpt_convert_cells(src, dst, nx, ny, nz, STORAGE_CODES[storage], stream), "dpt_convert_cells")
 
 
 def convert_field(src: int, dst: int, count: int, storage: str, status: int | None, stream: int) -> None:

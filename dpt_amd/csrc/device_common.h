@@ -23,6 +23,7 @@ constexpr int kMaxComponents = 12;
 
 struct Params {
     int nx, ny, nz;
+    int ix, iy;              // row / slab counts of the gather index: (nx, ny) linear, cell counts for cells
     int sx, sy, sz;          // corner strides: 1, nx, nx*ny, or 0 for singleton axes
     float hx, hy, hz;        // n - 0.5 (upper support bound)
     float cx, cy, cz;        // n - 1 (upper clamp)
@@ -260,8 +261,8 @@ struct Cell32 {
     float slx, sly, slz;     // slopes (0/1)
 };
 
-// Maximum ny * nz of a grid: cell indices use 24-bit multiplies
-// ((lz * ny + ly) * nx + lx with both products' operands below 2^24).
+// Maximum ny * nz of a grid: gather indices use 24-bit multiplies
+// ((lz * iy + ly) * ix + lx with both products' operands below 2^24).
 constexpr long long kMaxSliceProduct = 1LL << 24;
 
 // Support test of one FP32 sample point (kernels/projection_fp32.py:64-74):
@@ -288,7 +289,9 @@ __device__ __forceinline__ void cell32_inside(const Params &P, float p0, float p
     c.slx = (p0 >= 0.0f && p0 < P.cx) ? 1.0f : 0.0f;
     c.sly = (p1 >= 0.0f && p1 < P.cy) ? 1.0f : 0.0f;
     c.slz = (p2 >= 0.0f && p2 < P.cz) ? 1.0f : 0.0f;
-    c.base = (int)(__umul24(__umul24((unsigned)c.lz, (unsigned)P.ny) + (unsigned)c.ly, (unsigned)P.nx) +
+    // gather index: the voxel of the lower corner (linear layouts) or the
+    // cell (cell layout), by the row / slab counts of the launch
+    c.base = (int)(__umul24(__umul24((unsigned)c.lz, (unsigned)P.iy) + (unsigned)c.ly, (unsigned)P.ix) +
                    (unsigned)c.lx);
 }
 
@@ -357,10 +360,41 @@ __device__ __forceinline__ float load_global<__half>(global_const<__half> *f, in
     return __half2float(__ushort_as_half(bits));
 }
 
-template <typename T, bool PAIR>
+// Gather layouts of the FP32-sampling kernels.
+constexpr int kLayoutScalar = 0;  // linear field, eight scalar loads (nx == 1)
+constexpr int kLayoutPair = 1;    // linear field, x-adjacent pairs (nx > 1)
+constexpr int kLayoutCells = 2;   // cell layout (DPT_STORE_CELLS): one cell per gather
+
+struct __attribute__((aligned(16))) Quad {
+    uint32_t x, y, z, w;
+};
+
+template <typename T, int LAYOUT>
 __device__ __forceinline__ void load8(const Params &P, const T *field_any, int base, float *v) {
     global_const<T> *field = (global_const<T> *)field_any;
-    if constexpr (PAIR && std::is_same<T, float>::value) {
+    if constexpr (LAYOUT == kLayoutCells && std::is_same<T, float>::value) {
+        // 32-byte cell: corners 0-3 then 4-7, two 16-byte loads
+        global_const<Quad> *q = reinterpret_cast<global_const<Quad> *>(field) + 2 * (size_t)base;
+        const Quad a = {q[0].x, q[0].y, q[0].z, q[0].w};
+        const Quad b = {q[1].x, q[1].y, q[1].z, q[1].w};
+        v[0] = __uint_as_float(a.x);
+        v[1] = __uint_as_float(a.y);
+        v[2] = __uint_as_float(a.z);
+        v[3] = __uint_as_float(a.w);
+        v[4] = __uint_as_float(b.x);
+        v[5] = __uint_as_float(b.y);
+        v[6] = __uint_as_float(b.z);
+        v[7] = __uint_as_float(b.w);
+    } else if constexpr (LAYOUT == kLayoutCells) {
+        // 16-byte cell of 16-bit values, corner k in half k
+        global_const<Quad> *q = reinterpret_cast<global_const<Quad> *>(field) + (size_t)base;
+        const uint32_t w[4] = {q->x, q->y, q->z, q->w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[2 * j] = half_bits_to_float<T>(w[j] & 0xFFFFu);
+            v[2 * j + 1] = half_bits_to_float<T>(w[j] >> 16);
+        }
+    } else if constexpr (LAYOUT == kLayoutPair && std::is_same<T, float>::value) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             global_const<FloatPair> *q = reinterpret_cast<global_const<FloatPair> *>(
@@ -368,7 +402,7 @@ __device__ __forceinline__ void load8(const Params &P, const T *field_any, int b
             v[2 * j] = q->a;
             v[2 * j + 1] = q->b;
         }
-    } else if constexpr (PAIR) {
+    } else if constexpr (LAYOUT == kLayoutPair) {
         // 16-bit storage: the x-adjacent pair (i, i+1) lies in the dword-aligned
         // 8 bytes at dword i/2; one dwordx2 load and a funnel shift extract it.
         // Reads at most 4 bytes past element i+1: half-storage buffers carry

@@ -58,7 +58,7 @@ struct FitTable {
 // FP32 storage sums plainly (the projector's FP32 VJP arithmetic); 16-bit
 // storage uses blocked compensated sums.
 template <typename T, bool COMPENSATED>
-__device__ __forceinline__ void fit_ray(const Params &P, const T *field, const double *pose, int view, int row,
+__device__ __forceinline__ void fit_ray(const Params &P, const T *field, bool cells, const double *pose, int view, int row,
                                         int col, float &L, double *j, bool &valid) {
     World w;
     ray_world(P, pose, view, row, col, w);
@@ -114,10 +114,12 @@ __device__ __forceinline__ void fit_ray(const Params &P, const T *field, const d
             sample_at<false>(P, o, d, iv.lower, step, k, pos, c);
         }
         if (c.base != cur) {
-            if (P.sx)
-                load8<T, true>(P, field, c.base, v);
+            if (cells)
+                load8<T, kLayoutCells>(P, field, c.base, v);
+            else if (P.sx)
+                load8<T, kLayoutPair>(P, field, c.base, v);
             else
-                load8<T, false>(P, field, c.base, v);
+                load8<T, kLayoutScalar>(P, field, c.base, v);
             cur = c.base;
         }
         float wt[8], g[3];
@@ -190,13 +192,17 @@ __global__ __launch_bounds__(kBlock) void fit_step_kernel(const FitTable *__rest
             float L;
             double jr[6];
             bool valid;
-            if (R.storage == DPT_STORE_BF16)
-                fit_ray<__hip_bfloat16, true>(P, static_cast<const __hip_bfloat16 *>(R.field), R.state, view, row,
-                                              col, L, jr, valid);
-            else if (R.storage == DPT_STORE_F16)
-                fit_ray<__half, true>(P, static_cast<const __half *>(R.field), R.state, view, row, col, L, jr, valid);
+            const int element = R.storage & ~DPT_STORE_CELLS;
+            const bool cells = (R.storage & DPT_STORE_CELLS) != 0;
+            if (element == DPT_STORE_BF16)
+                fit_ray<__hip_bfloat16, true>(P, static_cast<const __hip_bfloat16 *>(R.field), cells, R.state, view,
+                                              row, col, L, jr, valid);
+            else if (element == DPT_STORE_F16)
+                fit_ray<__half, true>(P, static_cast<const __half *>(R.field), cells, R.state, view, row, col, L, jr,
+                                      valid);
             else
-                fit_ray<float, false>(P, static_cast<const float *>(R.field), R.state, view, row, col, L, jr, valid);
+                fit_ray<float, false>(P, static_cast<const float *>(R.field), cells, R.state, view, row, col, L, jr,
+                                      valid);
             const double res = (double)L - (double)R.targets[((size_t)view * P.rows + row) * P.cols + col];
             int q = 0;
 #pragma unroll
@@ -373,6 +379,11 @@ int fit_fail(int code, const std::string &msg) {
 }
 
 // Grid geometry of one request (same rules as dpt_plan_create).
+bool storage_ok(int storage) {
+    const int element = storage & ~DPT_STORE_CELLS;
+    return element == DPT_STORE_F32 || element == DPT_STORE_BF16 || element == DPT_STORE_F16;
+}
+
 bool fill_params(Params &P, const dpt_volume_desc &v, const dpt_fit_request &q, std::string &why) {
     if (v.nx < 1 || v.ny < 1 || v.nz < 1 || (int64_t)v.nx * v.ny * v.nz > INT32_MAX) {
         why = "volume counts must be positive and fit signed 32-bit indexing";
@@ -390,6 +401,10 @@ bool fill_params(Params &P, const dpt_volume_desc &v, const dpt_fit_request &q, 
     P.nx = v.nx;
     P.ny = v.ny;
     P.nz = v.nz;
+    // gather index: voxels of the linear field, or cells of the cell layout
+    const bool cells = (v.storage & DPT_STORE_CELLS) != 0;
+    P.ix = cells ? (v.nx > 1 ? v.nx - 1 : 1) : v.nx;
+    P.iy = cells ? (v.ny > 1 ? v.ny - 1 : 1) : v.ny;
     P.sx = v.nx > 1 ? 1 : 0;
     P.sy = v.ny > 1 ? v.nx : 0;
     P.sz = v.nz > 1 ? v.nx * v.ny : 0;
@@ -499,7 +514,7 @@ int dpt_fit_step(dpt_fit *f, const dpt_volume_desc *volumes, int32_t n_volumes, 
         const dpt_volume_desc &v = volumes[q.volume];
         FitRequestDev &R = h->req[r];
         if (!fill_params(R.P, v, q, why) || v.field == nullptr ||
-            (v.storage != DPT_STORE_F32 && v.storage != DPT_STORE_BF16 && v.storage != DPT_STORE_F16)) {
+            !storage_ok(v.storage)) {
             (void)hipSetDevice(previous);
             return fit_fail(DPT_E_INVALID, "request " + std::to_string(r) + ": " + (why.empty() ? "invalid volume" : why));
         }

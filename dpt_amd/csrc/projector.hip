@@ -43,7 +43,7 @@ namespace dpt {
 // ------------------------------------------------------- FP32 forward ------
 // kernels/projection_fp32.py:334-374
 
-template <typename T, bool COMPENSATED, bool PAIR>
+template <typename T, bool COMPENSATED, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__restrict__ field,
                                                            float *__restrict__ out) {
     int view, row, col, wslot;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
                     if (c.base < 0) continue;
                 }
                 if (c.base != cur) {
-                    load8<T, PAIR>(P, field, c.base, v);
+                    load8<T, LAYOUT>(P, field, c.base, v);
                     cur = c.base;
                 }
                 float wt[8];
@@ -251,7 +251,7 @@ __device__ __forceinline__ int wave_min(int v) {
 // brick entries to the FP64 accumulator with one global atomic each.  Chunks
 // whose brick does not fit use global atomics directly.
 
-template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, bool PAIR>
+template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
                                                        const float *__restrict__ seeds,
                                                        double *__restrict__ acc,
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     }
                 }
                 if (c.base != cur) {
-                    if constexpr (POSE) load8<T, PAIR>(P, field, c.base, v);
+                    if constexpr (POSE) load8<T, LAYOUT>(P, field, c.base, v);
                     if constexpr (VOL) {
                         if (bounded) {
                             if (use_lds)
@@ -462,7 +462,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                         for (int kk = 0; kk < 8; ++kk) {
                             const float inc = scatter_increment(volume_factor, wt[kk]);
                             if (!isfinite(inc)) bad = true;
-                            if (inc != 0.0f) unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)inc);
+                            // c.base indexes cells in the cell layout: address the voxel
+                            const int voxel = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+                            if (inc != 0.0f) unsafeAtomicAdd(&acc[voxel + corner_offset(P, kk)], (double)inc);
                         }
                     }
                 }
@@ -931,6 +933,34 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(const float *__restrict
     if (status != nullptr && __any(bad) && (threadIdx.x & 63) == 0) atomicOr(&status[0], DPT_STATUS_DOMAIN);
 }
 
+// Linear (nz, ny, nx) field -> cell layout (DPT_STORE_CELLS), one lane per
+// cell: the 8 corners of cell (i, j, k) in corner order (bit 0 x, bit 1 y,
+// bit 2 z), +1 neighbours clamped to the grid as the gather's zero strides
+// do on singleton axes.  E is the element's bit pattern (uint32_t for FP32,
+// uint16_t for BF16/FP16); a cell is written with 16-byte stores.
+template <typename E>
+__global__ __launch_bounds__(kBlock) void cells_kernel(const E *__restrict__ src, E *__restrict__ dst, int nx,
+                                                       int ny, int nz, int mx, int my, int cells) {
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < cells; c += gridDim.x * kBlock) {
+        const int i = c % mx, r = c / mx;
+        const int j = r % my, k = r / my;
+        const int i1 = min(i + 1, nx - 1), j1 = min(j + 1, ny - 1), k1 = min(k + 1, nz - 1);
+        const int64_t slab = (int64_t)nx * ny;
+        const int64_t a = k * slab + (int64_t)j * nx, b = k * slab + (int64_t)j1 * nx;
+        const int64_t e = k1 * slab + (int64_t)j * nx, f = k1 * slab + (int64_t)j1 * nx;
+        const uint32_t v[8] = {src[a + i], src[a + i1], src[b + i], src[b + i1],
+                               src[e + i], src[e + i1], src[f + i], src[f + i1]};
+        if constexpr (sizeof(E) == 4) {
+            Quad *q = reinterpret_cast<Quad *>(dst) + 2 * (size_t)c;
+            q[0] = Quad{v[0], v[1], v[2], v[3]};
+            q[1] = Quad{v[4], v[5], v[6], v[7]};
+        } else {
+            Quad *q = reinterpret_cast<Quad *>(dst) + (size_t)c;
+            *q = Quad{v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16)};
+        }
+    }
+}
+
 // Count rays whose FP32 clipped interval is non-empty (effective samples =
 // count * samples); same setup arithmetic as the forward kernel.
 __global__ __launch_bounds__(kBlock) void count_hits_kernel(Params P, unsigned long long *count) {
@@ -974,6 +1004,10 @@ struct dpt_plan {
     double *d_acc = nullptr;
     float *d_alpha = nullptr;
     unsigned long long *d_count = nullptr;
+    bool cells = false;       // FP32 sampling gathers from the cell-layout copy
+    int mx = 1, my = 1;       // cells along x and y
+    int64_t cell_count = 0;
+    void *d_cells = nullptr;  // 8 * cell_count elements of the storage type
     int vjp_vb = 1;          // views per workgroup of the VJP launches
     int forward_vb = 1;      // views per workgroup of the forward launch
     bool timing = false;
@@ -1081,17 +1115,27 @@ Params launch_params(const dpt_plan *p, const double *poses) {
 
 unsigned tile_grid(const Params &P) { return (unsigned)(P.chunk * kXcds); }
 
+// Launch parameters of a gather from the plan's cell-layout copy.
+Params cells_params(const dpt_plan *p, Params P) {
+    P.ix = p->mx;
+    P.iy = p->my;
+    return P;
+}
+
 template <typename T>
-int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, hipStream_t s) {
+int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, bool cells, hipStream_t s) {
     const T *f = static_cast<const T *>(field);
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         float *o = static_cast<float *>(out);
         constexpr bool HALF = !std::is_same<T, float>::value;
-        if (P.nx > 1)
-            hipLaunchKernelGGL((forward32_kernel<T, HALF, true>), grid, block, 0, s, P, f, o);
+        if (cells)
+            hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutCells>), grid, block, 0, s, cells_params(p, P),
+                               static_cast<const T *>(p->d_cells), o);
+        else if (P.nx > 1)
+            hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutPair>), grid, block, 0, s, P, f, o);
         else
-            hipLaunchKernelGGL((forward32_kernel<T, HALF, false>), grid, block, 0, s, P, f, o);
+            hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutScalar>), grid, block, 0, s, P, f, o);
     } else {
         hipLaunchKernelGGL((forward64_kernel<T>), grid, block, 0, s, P, f, static_cast<double *>(out));
     }
@@ -1099,33 +1143,36 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, h
     return DPT_OK;
 }
 
-template <typename T, int C, bool PAIR>
+template <typename T, int C, int LAYOUT>
 void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, hipStream_t s) {
     constexpr bool HALF = !std::is_same<T, float>::value;
     const dim3 grid(tile_grid(P)), block(kBlock);
     const size_t lds = (size_t)P.brick_cap * sizeof(double);
     if (vol && pose)
-        hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, PAIR>), grid, block, lds, s, P, f, sd, p->d_acc,
+        hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd, p->d_acc,
                            p->d_partials);
-    else if (vol)
-        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, PAIR>), grid, block, lds, s, P, f, sd, p->d_acc,
+    else if (pose)
+        hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, LAYOUT>), grid, block, 0, s, P, f, sd, p->d_acc,
                            p->d_partials);
-    else
-        hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, PAIR>), grid, block, 0, s, P, f, sd, p->d_acc,
-                           p->d_partials);
+    else if constexpr (LAYOUT != kLayoutCells)  // a volume-only VJP gathers nothing
+        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd,
+                           p->d_acc, p->d_partials);
 }
 
 template <typename T, int C>
 int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
-                 hipStream_t s) {
+                 bool cells, hipStream_t s) {
     const T *f = static_cast<const T *>(field);
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         const float *sd = static_cast<const float *>(seeds);
-        if (P.nx > 1)
-            launch_vjp32<T, C, true>(p, P, f, sd, vol, pose, s);
+        if (cells && pose)
+            launch_vjp32<T, C, kLayoutCells>(p, cells_params(p, P), static_cast<const T *>(p->d_cells), sd, vol,
+                                             pose, s);
+        else if (P.nx > 1)
+            launch_vjp32<T, C, kLayoutPair>(p, P, f, sd, vol, pose, s);
         else
-            launch_vjp32<T, C, false>(p, P, f, sd, vol, pose, s);
+            launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
     } else {
         const double *sd = static_cast<const double *>(seeds);
         if (vol && pose)
@@ -1144,9 +1191,29 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
 
 template <typename T>
 int launch_vjp(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
-               int components, hipStream_t s) {
-    if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, s);
-    return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, s);
+               int components, bool cells, hipStream_t s) {
+    if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, cells, s);
+    return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, cells, s);
+}
+
+int launch_cells(const void *src, void *dst, int nx, int ny, int nz, int storage, hipStream_t s) {
+    const int mx = nx > 1 ? nx - 1 : 1, my = ny > 1 ? ny - 1 : 1, mz = nz > 1 ? nz - 1 : 1;
+    const int64_t cells = (int64_t)mx * my * mz;
+    if (cells == 0) return DPT_OK;
+    if (storage == DPT_STORE_F32)
+        hipLaunchKernelGGL(cells_kernel<uint32_t>, dim3(grid_blocks(cells)), dim3(kBlock), 0, s,
+                           static_cast<const uint32_t *>(src), static_cast<uint32_t *>(dst), nx, ny, nz, mx, my,
+                           (int)cells);
+    else
+        hipLaunchKernelGGL(cells_kernel<uint16_t>, dim3(grid_blocks(cells)), dim3(kBlock), 0, s,
+                           static_cast<const uint16_t *>(src), static_cast<uint16_t *>(dst), nx, ny, nz, mx, my,
+                           (int)cells);
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+int refresh_cells(dpt_plan *p, const void *field, hipStream_t s) {
+    return launch_cells(field, p->d_cells, p->base.nx, p->base.ny, p->base.nz, p->storage, s);
 }
 
 void timer_begin(dpt_plan *p, int kind, hipStream_t s) {
@@ -1177,6 +1244,7 @@ int dpt_plan_destroy(dpt_plan *p) {
     (void)hipFree(p->d_partials);
     (void)hipFree(p->d_acc);
     (void)hipFree(p->d_alpha);
+    (void)hipFree(p->d_cells);
     (void)hipFree(p->d_count);
     if (p->h_status) (void)hipHostFree(p->h_status);
     for (auto &e : p->ev)
@@ -1224,6 +1292,8 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     P.nx = grid->nx;
     P.ny = grid->ny;
     P.nz = grid->nz;
+    P.ix = grid->nx;
+    P.iy = grid->ny;
     P.sx = grid->nx > 1 ? 1 : 0;
     P.sy = grid->ny > 1 ? grid->nx : 0;
     P.sz = grid->nz > 1 ? grid->nx * grid->ny : 0;
@@ -1259,6 +1329,15 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         const int forced = std::atoi(vb);
         if (!P.bundle && (forced == 1 || forced == 2 || forced == 4)) p->forward_vb = forced;
     }
+    p->cells = (needs & DPT_GATHER_CELLS) != 0;
+    if (const char *gather = std::getenv("DPT_AMD_GATHER")) {
+        if (std::strcmp(gather, "cells") == 0) p->cells = true;
+        if (std::strcmp(gather, "linear") == 0) p->cells = false;
+    }
+    p->cells = p->cells && arithmetic == DPT_ARITH_FP32;
+    p->mx = grid->nx > 1 ? grid->nx - 1 : 1;
+    p->my = grid->ny > 1 ? grid->ny - 1 : 1;
+    p->cell_count = (int64_t)p->mx * p->my * (grid->nz > 1 ? grid->nz - 1 : 1);
 
     auto cleanup = [&](int code, const std::string &msg) {
         dpt_plan_destroy(p);
@@ -1292,6 +1371,7 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         return cleanup(DPT_E_MEMORY, "pinned status allocation failed");
     if (needs & DPT_POSE) ALLOC(p->d_partials, sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * views);
     if (needs & DPT_VOLUME) ALLOC(p->d_acc, sizeof(double) * (size_t)voxels);
+    if (p->cells) ALLOC(p->d_cells, (storage == DPT_STORE_F32 ? 4 : 2) * 8 * (size_t)p->cell_count);
     ALLOC(p->d_count, sizeof(unsigned long long));
     ALLOC(p->d_alpha, sizeof(float) * sample_table_entries(samples));
     {
@@ -1330,14 +1410,18 @@ int dpt_forward(dpt_plan *p, const void *field, const double *poses, void *out, 
     }
     hipLaunchKernelGGL(check_poses_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, poses, P.views, p->d_status);
     HIP_TRY(hipGetLastError());
+    if (p->cells) {
+        const int rc = refresh_cells(p, field, s);
+        if (rc != DPT_OK) return rc;
+    }
     timer_begin(p, 0, s);
     int rc;
     if (p->storage == DPT_STORE_F32)
-        rc = launch_forward<float>(p, P, field, out, s);
+        rc = launch_forward<float>(p, P, field, out, p->cells, s);
     else if (p->storage == DPT_STORE_BF16)
-        rc = launch_forward<__hip_bfloat16>(p, P, field, out, s);
+        rc = launch_forward<__hip_bfloat16>(p, P, field, out, p->cells, s);
     else
-        rc = launch_forward<__half>(p, P, field, out, s);
+        rc = launch_forward<__half>(p, P, field, out, p->cells, s);
     timer_end(p, 0, s);
     return rc;
 }
@@ -1383,14 +1467,21 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     if (vol)
         hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc, p->voxels);
     HIP_TRY(hipGetLastError());
+    // the cell copy is rebuilt at every call: the field of this VJP may differ
+    // from the last forward's
+    const bool cells = p->cells && pose;
+    if (cells) {
+        const int rc = refresh_cells(p, field, s);
+        if (rc != DPT_OK) return rc;
+    }
     timer_begin(p, 1, s);
     int rc;
     if (p->storage == DPT_STORE_F32)
-        rc = launch_vjp<float>(p, P, field, cotangent, vol, pose, components, s);
+        rc = launch_vjp<float>(p, P, field, cotangent, vol, pose, components, cells, s);
     else if (p->storage == DPT_STORE_BF16)
-        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, vol, pose, components, s);
+        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, vol, pose, components, cells, s);
     else
-        rc = launch_vjp<__half>(p, P, field, cotangent, vol, pose, components, s);
+        rc = launch_vjp<__half>(p, P, field, cotangent, vol, pose, components, cells, s);
     timer_end(p, 1, s);
     if (rc != DPT_OK) return rc;
     const int accumulate = (flags & DPT_ACCUMULATE) ? 1 : 0;
@@ -1435,6 +1526,21 @@ int dpt_convert_field(const float *src, void *dst, int64_t count, int32_t storag
         return fail(DPT_E_INVALID, "storage must be DPT_STORE_BF16 or DPT_STORE_F16");
     HIP_TRY(hipGetLastError());
     return DPT_OK;
+}
+
+int64_t dpt_cell_count(int32_t nx, int32_t ny, int32_t nz) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(DPT_E_INVALID, "grid counts must be positive");
+    return (int64_t)(nx > 1 ? nx - 1 : 1) * (ny > 1 ? ny - 1 : 1) * (nz > 1 ? nz - 1 : 1);
+}
+
+int dpt_convert_cells(const void *src, void *dst, int32_t nx, int32_t ny, int32_t nz, int32_t storage,
+                      void *stream) {
+    if (src == nullptr || dst == nullptr) return fail(DPT_E_INVALID, "src and dst are required");
+    if (nx < 1 || ny < 1 || nz < 1 || (int64_t)nx * ny * nz > INT32_MAX)
+        return fail(DPT_E_INVALID, "grid counts must be positive and fit signed 32-bit indexing");
+    if (storage != DPT_STORE_F32 && storage != DPT_STORE_BF16 && storage != DPT_STORE_F16)
+        return fail(DPT_E_INVALID, "storage must be DPT_STORE_F32, DPT_STORE_BF16 or DPT_STORE_F16");
+    return launch_cells(src, dst, nx, ny, nz, storage, static_cast<hipStream_t>(stream));
 }
 
 int dpt_set_timing(dpt_plan *p, int32_t enabled) {

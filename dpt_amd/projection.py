@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 import threading
 from collections import OrderedDict
 from dataclasses import dataclass, field
@@ -159,6 +160,21 @@ def _check_extent(decoded: Decoded) -> None:
                     raise ContractError("detector extent exceeds the FP64 finite-ray range")
 
 
+# Largest cell-layout copy (bytes) a plan keeps by default; larger fields
+# gather from the linear array.
+CELL_GATHER_MAX_BYTES = 32 << 30
+
+
+def gather_cells(decoded: Decoded) -> bool:
+    """FP32 sampling gathers corners from a cell-layout copy of the field
+    (one or two 16-byte loads per cell instead of four 8-byte loads) unless
+    DPT_AMD_GATHER=linear or the copy would exceed CELL_GATHER_MAX_BYTES."""
+    if decoded.spec.precision != "float32" or os.environ.get("DPT_AMD_GATHER") == "linear":
+        return False
+    element = 4 if decoded.spec.storage == "float32" else 2
+    return 8 * element * _native.cell_count(decoded.grid.shape) <= CELL_GATHER_MAX_BYTES
+
+
 def prepare(value: str, device: int, stream: int, needs: int) -> Prepared:
     """Return the cached plan for this configuration, device, stream and flags."""
     key = (value, int(device), int(stream), int(needs))
@@ -169,6 +185,8 @@ def prepare(value: str, device: int, stream: int, needs: int) -> Prepared:
             return found
         decoded = decode(value)
         _check_extent(decoded)
+        if gather_cells(decoded):
+            needs |= _native.GATHER_CELLS
         plan = _native.Plan(
             decoded.grid,
             samples=decoded.spec.samples_per_ray,

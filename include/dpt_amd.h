@@ -48,6 +48,18 @@ extern "C" {
 /* BF16/FP16 field buffers must stay readable for this many bytes past the
  * last voxel (x-adjacent pairs are fetched with one 8-byte load). */
 #define DPT_HALF_PAD_BYTES 8
+/* Cell layout, OR-ed into a DPT_STORE_* code of a resident (fit) volume:
+ * the field is stored as interpolation cells, cell (i, j, k) for
+ * i < max(nx-1, 1), j < max(ny-1, 1), k < max(nz-1, 1) (x fastest) holding
+ * its 8 corner values contiguously, corner q = (x + (q&1), y + (q>>1&1),
+ * z + (q>>2&1)) clamped to the grid.  One cell is 32 bytes (FP32) or 16
+ * bytes (BF16/FP16): a trilinear sample is one or two 16-byte loads.  Build
+ * it with dpt_convert_cells. */
+#define DPT_STORE_CELLS 8
+/* dpt_plan_create `needs` option: FP32 sampling launches gather corners
+ * from a plan-owned cell-layout copy of the field, rebuilt from the linear
+ * field at each dpt_forward / dpt_vjp call (the caller's API is unchanged). */
+#define DPT_GATHER_CELLS 64u
 
 /* dpt_vjp flags */
 #define DPT_VOLUME 1u        /* produce grad_field */
@@ -131,6 +143,18 @@ int dpt_status(dpt_plan *plan, int32_t *host_codes, int32_t clear, void *stream)
  * counterpart: the reference has no FP32-sampling half-storage form
  * (python/dpt/projection.py:131-136 forbids it). */
 int dpt_convert_field(const float *src, void *dst, int64_t count, int32_t storage, int32_t *status,
+                      void *stream);
+
+/* Cells of a grid in the DPT_STORE_CELLS layout:
+ * max(nx-1,1) * max(ny-1,1) * max(nz-1,1); each holds 8 values. */
+int64_t dpt_cell_count(int32_t nx, int32_t ny, int32_t nz);
+
+/* Rewrite a linear (nz,ny,nx) field of `storage` (DPT_STORE_F32/BF16/F16)
+ * into the DPT_STORE_CELLS layout of the same element type: dst holds
+ * 8 * dpt_cell_count(nx, ny, nz) elements.  Enqueued on `stream`.  No
+ * reference counterpart: the reference gathers the 8 corners from the linear
+ * array (kernels/projection_fp32.py:157-168). */
+int dpt_convert_cells(const void *src, void *dst, int32_t nx, int32_t ny, int32_t nz, int32_t storage,
                       void *stream);
 
 /* Kernel time of the most recent launch of `kind` (0 forward, 1 VJP) on the
