@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -106,8 +107,12 @@ def kernels_once(args) -> int:
     grad_field = torch.empty(grid.shape, device="cuda:0")
     grad_pose = torch.empty((args.views, 12), dtype=torch.float64, device="cuda:0")
     stream = torch.cuda.current_stream().cuda_stream
+    # the projector's default FP32 plan gathers from the cell layout
+    # (DPT_AMD_GATHER=linear selects the linear gathers)
+    cells = os.environ.get("DPT_AMD_GATHER") != "linear"
+    needs = _native.VOLUME | _native.POSE | (_native.GATHER_CELLS if cells else 0)
     plan = _native.Plan(grid, views=args.views, rows=DETECTOR[0], cols=DETECTOR[1], panels=panels, rays=None,
-                        samples=args.samples, arithmetic="float32", storage="float32", needs=3, device=0)
+                        samples=args.samples, arithmetic="float32", storage="float32", needs=needs, device=0)
     plan.forward(field.data_ptr(), poses.data_ptr(), out.data_ptr(), stream)
     seed = (out - 0.5 * out.mean()).contiguous()
     for flags in (_native.VOLUME, _native.POSE, _native.VOLUME | _native.POSE):
