@@ -146,8 +146,10 @@ struct Brick {
     int volume;
 };
 
-// Add the non-zero values f[k] at the eight corners of the register cell:
-// into the LDS brick (FP64 LDS atomics) or into the FP64 global accumulator.
+// Add the values f[k] at the eight corners of the register cell: into the
+// LDS brick (FP64 LDS atomics, every corner: a lane's cell sums are non-zero
+// but for exact-boundary fractions) or into the FP64 global accumulator
+// (non-zero values only: device atomics are the expensive kind).
 template <bool LDS>
 __device__ __forceinline__ void add_corners(const Params &P, const Brick &B, const CellSums &cs, const float *f,
                                             double *brick, double *acc) {
@@ -158,7 +160,7 @@ __device__ __forceinline__ void add_corners(const Params &P, const Brick &B, con
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
-            if (f[k] != 0.0f) atomicAdd(&brick[base + off], (double)f[k]);
+            atomicAdd(&brick[base + off], (double)f[k]);
         }
     } else {
         const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
@@ -179,7 +181,9 @@ __device__ __forceinline__ void flush(const Params &P, const Brick &B, CellSums 
 // (kernels/projection_fp32.py:272-331): when the new cell is the old one
 // stepped by +-1 along one axis, the four corners on the shared face stay in
 // registers (renamed to the opposite side of the new cell) and only the four
-// departing corners are flushed; any other move flushes all eight.
+// departing corners are flushed; any other move flushes all eight.  Used for
+// chunks whose brick does not fit LDS, whose flushes are device atomics (the
+// expensive kind): the carry halves them.
 //
 // Inside a lockstep sample loop some lane of a wave changes cell at nearly
 // every sample, so this code runs at wave level once per sample: it is
@@ -187,8 +191,8 @@ __device__ __forceinline__ void flush(const Params &P, const Brick &B, CellSums 
 // per-corner atomic masks).  An empty register cell (x < 0) holds zeros, so
 // it flushes nothing and carries zeros.
 template <bool LDS>
-__device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
-                                           double *brick, double *acc) {
+__device__ __forceinline__ void transition_carry(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
+                                                 double *brick, double *acc) {
     const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
     const bool live = cs.x >= 0;
     const bool on_x = live && dy == 0 && dz == 0, on_y = live && dx == 0 && dz == 0,
@@ -219,6 +223,26 @@ __device__ __forceinline__ void transition(const Params &P, const Brick &B, Cell
     cs.x = c.lx;
     cs.y = c.ly;
     cs.z = c.lz;
+}
+
+// Cell change.  Into an LDS brick the register cell sums
+// (kernels/projection_fp32.py:261-269) are flushed whole and the new cell
+// starts from zero: the carry's 24 selects and renames, run at wave level
+// once per sample, cost a third of the kernel's VALU, while the FP64 LDS
+// atomics it saves are cheap.  The result differs from the carried form
+// only in how the FP32 partial sums of a shared corner are split before
+// their exact FP64 addition.
+template <bool LDS>
+__device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
+                                           double *brick, double *acc) {
+    if constexpr (LDS) {
+        flush<true>(P, B, cs, brick, acc);
+        cs.x = c.lx;
+        cs.y = c.ly;
+        cs.z = c.lz;
+    } else {
+        transition_carry<false>(P, B, cs, c, brick, acc);
+    }
 }
 
 constexpr int kChunk = 32;         // samples per lockstep chunk
