@@ -144,6 +144,7 @@ struct Brick {
     int bx, by, bz;      // extents
     int lsx, lsy, lsz;   // local corner strides (0 on singleton grid axes)
     int volume;
+    int origin;          // (z0 * by + y0) * bx + x0: local index = (z * by + y) * bx + x - origin
 };
 
 // Add the values f[k] at the eight corners of the register cell: into the
@@ -154,9 +155,10 @@ template <bool LDS>
 __device__ __forceinline__ void add_corners(const Params &P, const Brick &B, const CellSums &cs, const float *f,
                                             double *brick, double *acc) {
     if constexpr (LDS) {
-        const int base = (int)(__umul24(__umul24((unsigned)(cs.z - B.z0), (unsigned)B.by) + (unsigned)(cs.y - B.y0),
-                                        (unsigned)B.bx) +
-                               (unsigned)(cs.x - B.x0));
+        // (z * by + y) * bx + x < ny * nz * nx: operands below 2^24 (grid check)
+        const int base =
+            (int)(__umul24(__umul24((unsigned)cs.z, (unsigned)B.by) + (unsigned)cs.y, (unsigned)B.bx) + (unsigned)cs.x) -
+            B.origin;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
@@ -395,10 +397,14 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     B.bx = __builtin_amdgcn_readfirstlane(e[0]);
                     B.by = __builtin_amdgcn_readfirstlane(e[1]);
                     B.bz = __builtin_amdgcn_readfirstlane(e[2]);
-                    B.lsx = P.nx > 1 ? 1 : 0;
+                    // x-pair and cell layouts run only on grids with nx > 1:
+                    // a compile-time unit x stride lets the LDS atomics of
+                    // x-adjacent corners use the instruction's offset field
+                    B.lsx = (LAYOUT != kLayoutScalar || P.nx > 1) ? 1 : 0;
                     B.lsy = P.ny > 1 ? B.bx : 0;
                     B.lsz = P.nz > 1 ? B.bx * B.by : 0;
                     B.volume = B.bx * B.by * B.bz;
+                    B.origin = (B.z0 * B.by + B.y0) * B.bx + B.x0;
                 }
             }
         }
@@ -1190,7 +1196,7 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         const float *sd = static_cast<const float *>(seeds);
-        if (cells && pose)
+        if (cells && pose && P.nx > 1)
             launch_vjp32<T, C, kLayoutCells>(p, cells_params(p, P), static_cast<const T *>(p->d_cells), sd, vol,
                                              pose, s);
         else if (P.nx > 1)
