@@ -197,6 +197,15 @@ def serving_benchmark(count: int, rate: float) -> dict:
         engine.run(requests[:2], realtime=False)  # warm-up: code objects, graphs
         closed = engine.run(requests, realtime=False)
         opened = engine.run(timed, realtime=True)
+        sweep = {}
+        if name == "batched_bf16":
+            # latency across the arrival rates of the served workload (5-200 req/s)
+            for r in (5.0, 50.0, 200.0):
+                at = serving.poisson_arrivals(count, r, seed=406)
+                run = engine.run([dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, at)],
+                                 realtime=True)
+                sweep[f"{r:g}_per_s"] = {k: run.summary()[k] for k in ("solves_per_s", "p50_latency_ms",
+                                                                       "p99_latency_ms", "peak_active")}
         errors = []
         for result in closed.results:
             k = int(result.key[3:])
@@ -209,6 +218,7 @@ def serving_benchmark(count: int, rate: float) -> dict:
             "graph": graph,
             "closed_loop": closed.summary(),
             "open_loop": opened.summary(),
+            **({"open_loop_rates": sweep} if sweep else {}),
             "median_rotation_error_deg": round(float(np.median(errors[:, 0])), 5),
             "median_translation_error_mm": round(float(np.median(errors[:, 1])), 5),
         }

@@ -53,6 +53,20 @@ def test_library_rejects_invalid_plans_without_a_device():
     assert rc == -1 and b"samples" in lib.dpt_last_error()
     rc = lib.dpt_plan_create(ctypes.byref(desc), 1, 1, 1, None, None, None, 8, 7, 0, 0, 0, ctypes.byref(out))
     assert rc == -1 and b"arithmetic" in lib.dpt_last_error()
+    # gather indices use 24-bit multiplies: ny * nz must not exceed 2^24
+    desc.nx, desc.ny, desc.nz = 2, 4097, 4097
+    rc = lib.dpt_plan_create(ctypes.byref(desc), 1, 1, 1, None, None, None, 8, 0, 0, 0, 0, ctypes.byref(out))
+    assert rc == -1 and b"2^24" in lib.dpt_last_error()
+
+
+def test_cell_layout_counts_and_argument_checks_without_a_device():
+    assert _native.cell_count((5, 6, 7)) == 4 * 5 * 6
+    assert _native.cell_count((1, 1, 9)) == 8
+    assert _native.cell_count((1, 1, 1)) == 1
+    lib = _native.lib()
+    assert lib.dpt_cell_count(0, 1, 1) == -1
+    assert lib.dpt_convert_cells(None, None, 4, 4, 4, 0, None) == -1
+    assert b"src and dst" in lib.dpt_last_error()
 
 
 @pytest.mark.parametrize(
