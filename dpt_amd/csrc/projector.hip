@@ -247,7 +247,16 @@ __device__ __forceinline__ void transition(const Params &P, const Brick &B, Cell
     }
 }
 
-constexpr int kChunk = 32;         // samples per lockstep chunk
+// Samples per lockstep chunk of the volume adjoint: 32 at 1024 samples over
+// a 256^3 grid (measured best against 16/24/48/64), scaled so that a chunk
+// spans about the same distance in voxels -- 32 * (S / 1024) * (443 / d),
+// d = the grid diagonal in voxels -- within [8, 64]: sparser sampling covers
+// more voxels per sample and would otherwise grow the chunk bricks past LDS.
+inline int chunk_samples(int samples, int nx, int ny, int nz) {
+    const double diagonal = std::sqrt((double)nx * nx + (double)ny * ny + (double)nz * nz);
+    const double chunk = 32.0 * samples / 1024.0 * 443.4 / diagonal;
+    return (int)std::lround(std::fmin(64.0, std::fmax(8.0, chunk)));
+}
 // FP64 LDS brick entries per workgroup (dynamic LDS, 8 B each).  gfx950
 // executes ds_add_f64 about ten times faster than ds_add_f32 (20 vs 193
 // cycles per wave64 instruction, measured), so the brick accumulates the FP32
@@ -341,10 +350,10 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     if constexpr (VOL) {
         for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
     }
-    const int chunks = (P.samples + kChunk - 1) / kChunk;
+    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
     for (int ch = 0; ch < chunks; ++ch) {
-        const int k0 = ch * kChunk;
-        const int k1 = min(k0 + kChunk, P.samples);
+        const int k0 = ch * P.chunk_samples;
+        const int k1 = min(k0 + P.chunk_samples, P.samples);
         Brick B{};
         bool use_lds = false;
         if constexpr (VOL) {
@@ -1345,6 +1354,11 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     P.bundle = panels == nullptr ? 1 : 0;
     configure_tiles(P, 1);
     P.brick_cap = kBrickCapDefault;
+    P.chunk_samples = chunk_samples(samples, grid->nx, grid->ny, grid->nz);
+    if (const char *chunk = std::getenv("DPT_AMD_CHUNK")) {
+        const int forced = std::atoi(chunk);
+        if (forced >= 1 && forced <= 1024) P.chunk_samples = forced;
+    }
     if (const char *cap = std::getenv("DPT_AMD_BRICK_CAP")) {
         const int forced = std::atoi(cap);
         if (forced >= 512 && forced <= 8192) P.brick_cap = forced;
