@@ -277,6 +277,77 @@ __device__ __forceinline__ int wave_min(int v) {
     return v;
 }
 
+// Bound one chunk's cells over the workgroup and, when the box fits the LDS
+// brick, describe it.  lo / nhi: this lane's lower cell corner minimum and
+// negated maximum over its chunk samples (INT_MAX when it has none).  Every
+// lane gets the same answer (kept in SGPRs); one barrier.
+__device__ __forceinline__ bool chunk_brick(const Params &P, int (*red)[6], int *lo, int *nhi, bool unit_x,
+                                            Brick &B) {
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        nhi[a] = wave_min(nhi[a]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            red[wv][a] = lo[a];
+            red[wv][3 + a] = nhi[a];
+        }
+    }
+    __syncthreads();
+    int l[3], h[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        l[a] = min(min(red[0][a], red[1][a]), min(red[2][a], red[3][a]));
+        h[a] = -min(min(red[0][3 + a], red[1][3 + a]), min(red[2][3 + a], red[3][3 + a]));
+    }
+    if (l[0] == INT_MAX) return false;
+    const int n[3] = {P.nx, P.ny, P.nz};
+    int e[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
+    const long long vol = (long long)e[0] * e[1] * e[2];
+    if (vol > P.brick_cap) return false;
+    // every lane holds the same values: keep them in SGPRs
+    B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
+    B.y0 = __builtin_amdgcn_readfirstlane(l[1]);
+    B.z0 = __builtin_amdgcn_readfirstlane(l[2]);
+    B.bx = __builtin_amdgcn_readfirstlane(e[0]);
+    B.by = __builtin_amdgcn_readfirstlane(e[1]);
+    B.bz = __builtin_amdgcn_readfirstlane(e[2]);
+    B.lsx = (unit_x || P.nx > 1) ? 1 : 0;
+    B.lsy = P.ny > 1 ? B.bx : 0;
+    B.lsz = P.nz > 1 ? B.bx * B.by : 0;
+    B.volume = B.bx * B.by * B.bz;
+    B.origin = (B.z0 * B.by + B.y0) * B.bx + B.x0;
+    return true;
+}
+
+// Add the brick's non-zero entries to the FP64 accumulator (one device
+// atomic per touched grid sample) and clear them.  Call after a barrier.
+__device__ __forceinline__ void drain_brick(const Params &P, const Brick &B, double *brick, double *acc) {
+    const float inv_bx = 1.0f / (float)B.bx, inv_by = 1.0f / (float)B.by;
+    for (int i = threadIdx.x; i < B.volume; i += kBlock) {
+        const double v = brick[i];
+        if (v != 0.0) {
+            brick[i] = 0.0;
+            // i = (z * by + y) * bx + x: float quotients corrected by one step
+            int q = (int)((float)i * inv_bx);
+            q -= (q * B.bx > i) ? 1 : 0;
+            q += ((q + 1) * B.bx <= i) ? 1 : 0;
+            const int x = i - q * B.bx;
+            int z = (int)((float)q * inv_by);
+            z -= (z * B.by > q) ? 1 : 0;
+            z += ((z + 1) * B.by <= q) ? 1 : 0;
+            const int y = q - z * B.by;
+            const int g = ((B.z0 + z) * P.ny + (B.y0 + y)) * P.nx + (B.x0 + x);
+            unsafeAtomicAdd(&acc[g], v);
+        }
+    }
+}
+
 // ---------------------------------------------------------- FP32 VJP ------
 // kernels/projection_fp32.py:377-617.  The workgroup walks its rays' samples
 // in lockstep chunks.  Per chunk it bounds the cells its rays touch (sample
@@ -371,51 +442,10 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     nhi[a] = -max(la[a], lb[a]);
                 }
             }
-            const int wv = threadIdx.x >> 6;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = wave_min(lo[a]);
-                nhi[a] = wave_min(nhi[a]);
-            }
-            if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    bound_red[wv][a] = lo[a];
-                    bound_red[wv][3 + a] = nhi[a];
-                }
-            }
-            __syncthreads();
-            int l[3], h[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                l[a] = min(min(bound_red[0][a], bound_red[1][a]), min(bound_red[2][a], bound_red[3][a]));
-                h[a] = -min(min(bound_red[0][3 + a], bound_red[1][3 + a]), min(bound_red[2][3 + a], bound_red[3][3 + a]));
-            }
-            if (l[0] != INT_MAX) {
-                const int n[3] = {P.nx, P.ny, P.nz};
-                int e[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
-                const long long vol = (long long)e[0] * e[1] * e[2];
-                if (vol <= P.brick_cap) {
-                    // every lane holds the same values: keep them in SGPRs
-                    use_lds = true;
-                    B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
-                    B.y0 = __builtin_amdgcn_readfirstlane(l[1]);
-                    B.z0 = __builtin_amdgcn_readfirstlane(l[2]);
-                    B.bx = __builtin_amdgcn_readfirstlane(e[0]);
-                    B.by = __builtin_amdgcn_readfirstlane(e[1]);
-                    B.bz = __builtin_amdgcn_readfirstlane(e[2]);
-                    // x-pair and cell layouts run only on grids with nx > 1:
-                    // a compile-time unit x stride lets the LDS atomics of
-                    // x-adjacent corners use the instruction's offset field
-                    B.lsx = (LAYOUT != kLayoutScalar || P.nx > 1) ? 1 : 0;
-                    B.lsy = P.ny > 1 ? B.bx : 0;
-                    B.lsz = P.nz > 1 ? B.bx * B.by : 0;
-                    B.volume = B.bx * B.by * B.bz;
-                    B.origin = (B.z0 * B.by + B.y0) * B.bx + B.x0;
-                }
-            }
+            // x-pair and cell layouts run only on grids with nx > 1: a
+            // compile-time unit x stride lets the LDS atomics of x-adjacent
+            // corners use the instruction's offset field
+            use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B);
         }
         if (active) {
             // Lockstep walk of this chunk with the current cell cached (see
@@ -524,24 +554,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             if (use_lds) {
                 flush<true>(P, B, cs, brick, acc);
                 __syncthreads();
-                // drain the brick: one FP64 global atomic per touched sample
-                const float inv_bx = 1.0f / (float)B.bx, inv_by = 1.0f / (float)B.by;
-                for (int i = threadIdx.x; i < B.volume; i += kBlock) {
-                    const double v = brick[i];
-                    if (v != 0.0) {
-                        brick[i] = 0.0;
-                        int q = (int)((float)i * inv_bx);
-                        q -= (q * B.bx > i) ? 1 : 0;
-                        q += ((q + 1) * B.bx <= i) ? 1 : 0;
-                        const int x = i - q * B.bx;
-                        int z = (int)((float)q * inv_by);
-                        z -= (z * B.by > q) ? 1 : 0;
-                        z += ((z + 1) * B.by <= q) ? 1 : 0;
-                        const int y = q - z * B.by;
-                        const int g = ((B.z0 + z) * P.ny + (B.y0 + y)) * P.nx + (B.x0 + x);
-                        unsafeAtomicAdd(&acc[g], v);
-                    }
-                }
+                drain_brick(P, B, brick, acc);
             } else {
                 flush<false>(P, B, cs, brick, acc);
             }
@@ -625,6 +638,7 @@ __device__ __forceinline__ Interval64 interval64(const Params &P, const double *
 
 struct Cell64 {
     int base;
+    int lx, ly, lz;
     double fx, fy, fz;
     double slx, sly, slz;
 };
@@ -641,6 +655,9 @@ __device__ __forceinline__ bool cell64(const Params &P, const double *p, Cell64 
     const int lx = min((int)floor(v0), P.capx);
     const int ly = min((int)floor(v1), P.capy);
     const int lz = min((int)floor(v2), P.capz);
+    c.lx = lx;
+    c.ly = ly;
+    c.lz = lz;
     c.fx = v0 - (double)lx;
     c.fy = v1 - (double)ly;
     c.fz = v2 - (double)lz;
@@ -649,6 +666,14 @@ __device__ __forceinline__ bool cell64(const Params &P, const double *p, Cell64 
     c.slz = (p[2] >= 0.0 && p[2] < mz) ? 1.0 : 0.0;
     c.base = (lz * P.ny + ly) * P.nx + lx;
     return true;
+}
+
+// Clamped lower cell of an FP64 sample point (cell64's arithmetic without the
+// support test: out-of-support samples only widen a chunk's box).
+__device__ __forceinline__ void lower_cell64(const Params &P, const double *p, int *l) {
+    l[0] = min((int)floor(fmin(fmax(p[0], 0.0), (double)(P.nx - 1))), P.capx);
+    l[1] = min((int)floor(fmin(fmax(p[1], 0.0), (double)(P.ny - 1))), P.capy);
+    l[2] = min((int)floor(fmin(fmax(p[2], 0.0), (double)(P.nz - 1))), P.capz);
 }
 
 // kernels/projection.py:236-274 (weighted lerps; differences before interpolation)
@@ -752,6 +777,8 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                                                        const double *__restrict__ seeds,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ partials) {
+    extern __shared__ double brick[];
+    __shared__ int bound_red[kBlock / 64][6];
     int view, row, col, wslot;
     bool wave_valid;
     if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
@@ -759,103 +786,159 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
 #pragma unroll
     for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
     const double *pose = P.poses + 12 * (size_t)view;
+
+    // ---- per-ray setup
+    bool active = false;
+    Ray r{};
+    Interval64 iv{};
+    double span = 0.0, seed = 0.0, factor = 0.0, step = 0.0, volume_factor = 0.0;
+    const double S = (double)P.samples;
     if (ray_in_range(P, view, row, col)) {
         const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
         World w;
         ray_world(P, pose, view, row, col, w);
-        Ray r;
         ray_grid(P, w, r);
         if (!r.valid) {
             flag(P, view, DPT_STATUS_RANGE);
         } else {
-            const Interval64 iv = interval64(P, r.o, r.d);
-            double span = iv.upper - iv.lower;
-            const double seed = seeds[ray];
+            iv = interval64(P, r.o, r.d);
+            span = iv.upper - iv.lower;
+            seed = seeds[ray];
             if (seed == 0.0) span = 0.0;
             if (span > 0.0) {
-                double factor = seed * r.len;
-                const double S = (double)P.samples;
-                const double step = span / S;
-                double volume_factor = factor * span / S;
+                active = true;
+                factor = seed * r.len;
+                step = span / S;
+                volume_factor = factor * span / S;
                 if (!isfinite(factor)) volume_factor = product4(seed, r.len, span, 1.0 / S);
-                const bool pow2 = (P.samples & (P.samples - 1)) == 0;
-                const double inverse_samples = pow2 ? 1.0 / S : 0.0;
-                double mean = 0.0, dor[3] = {0, 0, 0}, ddir[3] = {0, 0, 0};
-                double shift_lower = 0.0, shift_upper = 0.0;
-                bool bad = false;
-                for (int k = 0; k < P.samples; ++k) {
-                    const double alpha = pow2 ? ((double)k + 0.5) * inverse_samples : ((double)k + 0.5) / S;
-                    const double pos = iv.lower + ((double)k + 0.5) * step;
-                    const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1],
-                                         r.o[2] + pos * r.d[2]};
-                    bad |= !finite3d(p);
-                    Cell64 c;
-                    if (!cell64(P, p, c)) continue;
-                    if constexpr (POSE) {
-                        double v[8], g[3];
-                        load8d<T>(P, field, c.base, v);
-                        const double value = sample_cell64<true>(c, v, g);
-                        mean += value;
+            }
+        }
+    }
+    const bool pow2 = (P.samples & (P.samples - 1)) == 0;
+    const double inverse_samples = pow2 ? 1.0 / S : 0.0;
+    double mean = 0.0, dor[3] = {0, 0, 0}, ddir[3] = {0, 0, 0};
+    double shift_lower = 0.0, shift_upper = 0.0;
+    bool bad = false;
+
+    // Lockstep chunks with LDS bricks, as vjp32_kernel: the reference adds
+    // every FP32-rounded contribution with its own atomic; here they go to
+    // the workgroup's FP64 LDS brick (exact sums of FP32 values) and reach
+    // the FP64 accumulator once per touched grid sample and chunk.
+    if constexpr (VOL) {
+        for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
+    }
+    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
+    for (int ch = 0; ch < chunks; ++ch) {
+        const int k0 = ch * P.chunk_samples;
+        const int k1 = min(k0 + P.chunk_samples, P.samples);
+        Brick B{};
+        bool use_lds = false;
+        if constexpr (VOL) {
+            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+            if (active) {
+                // sample positions are monotone in k: the chunk's end samples bound it
+                const double pa = iv.lower + ((double)k0 + 0.5) * step;
+                const double pb = iv.lower + ((double)(k1 - 1) + 0.5) * step;
+                const double qa[3] = {r.o[0] + pa * r.d[0], r.o[1] + pa * r.d[1], r.o[2] + pa * r.d[2]};
+                const double qb[3] = {r.o[0] + pb * r.d[0], r.o[1] + pb * r.d[1], r.o[2] + pb * r.d[2]};
+                int la[3], lb[3];
+                lower_cell64(P, qa, la);
+                lower_cell64(P, qb, lb);
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            dor[a] += g[a];
-                            ddir[a] += pos * g[a];
-                        }
-                        const double along = g[0] * r.d[0] + g[1] * r.d[1] + g[2] * r.d[2];
-                        shift_lower += (1.0 - alpha) * along;
-                        shift_upper += alpha * along;
-                    }
-                    if constexpr (VOL) {
-                        // kernels/projection.py:396-430: seed * weights in axis order, one FP32 rounding
-#pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) {
-                            double contribution = volume_factor;
-                            contribution *= (kk & 1) ? c.fx : 1.0 - c.fx;
-                            contribution *= (kk & 2) ? c.fy : 1.0 - c.fy;
-                            contribution *= (kk & 4) ? c.fz : 1.0 - c.fz;
-                            const float rounded = (float)contribution;
-                            if (!isfinite(rounded)) bad = true;
-                            if (rounded != 0.0f)
-                                unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)rounded);
-                        }
-                    }
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = min(la[a], lb[a]);
+                    nhi[a] = -max(la[a], lb[a]);
                 }
-                if (bad) flag(P, view, DPT_STATUS_RANGE);
+            }
+            use_lds = chunk_brick(P, bound_red, lo, nhi, false, B);
+        }
+        if (active) {
+            for (int k = k0; k < k1; ++k) {
+                const double alpha = pow2 ? ((double)k + 0.5) * inverse_samples : ((double)k + 0.5) / S;
+                const double pos = iv.lower + ((double)k + 0.5) * step;
+                const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1], r.o[2] + pos * r.d[2]};
+                bad |= !finite3d(p);
+                Cell64 c;
+                if (!cell64(P, p, c)) continue;
                 if constexpr (POSE) {
-                    const double lower_seed = -mean + span * shift_lower;
-                    const double upper_seed = mean + span * shift_upper;
-                    double og[3], dg[3];
+                    double v[8], g[3];
+                    load8d<T>(P, field, c.base, v);
+                    const double value = sample_cell64<true>(c, v, g);
+                    mean += value;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0;
-                        const double lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
-                        const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0;
-                        const double up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
-                        og[a] = span * dor[a] + lower_seed * lo_o + upper_seed * up_o;
-                        dg[a] = span * ddir[a] + lower_seed * lo_d + upper_seed * up_d;
+                        dor[a] += g[a];
+                        ddir[a] += pos * g[a];
                     }
-                    factor /= S;
-                    if (isfinite(factor)) {
-                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient);
-                    } else {
-                        // kernels/projection.py:1163-1183 range-safe products
-                        double sa[3], da[3];
+                    const double along = g[0] * r.d[0] + g[1] * r.d[1] + g[2] * r.d[2];
+                    shift_lower += (1.0 - alpha) * along;
+                    shift_upper += alpha * along;
+                }
+                if constexpr (VOL) {
+                    // kernels/projection.py:396-430: seed * weights in axis order, one FP32 rounding
+                    const int local = use_lds ? ((c.lz - B.z0) * B.by + (c.ly - B.y0)) * B.bx + (c.lx - B.x0) : 0;
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            const double s = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
-                            const double t = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
-                            sa[i] = product4(seed, r.len, 1.0 / S, s);
-                            da[i] = product4(seed, r.len, 1.0 / S, t);
+                    for (int kk = 0; kk < 8; ++kk) {
+                        double contribution = volume_factor;
+                        contribution *= (kk & 1) ? c.fx : 1.0 - c.fx;
+                        contribution *= (kk & 2) ? c.fy : 1.0 - c.fy;
+                        contribution *= (kk & 4) ? c.fz : 1.0 - c.fz;
+                        const float rounded = (float)contribution;
+                        if (!isfinite(rounded)) bad = true;
+                        if (rounded == 0.0f) continue;
+                        if (use_lds) {
+                            const int off = ((kk & 1) ? B.lsx : 0) + ((kk & 2) ? B.lsy : 0) + ((kk & 4) ? B.lsz : 0);
+                            atomicAdd(&brick[local + off], (double)rounded);
+                        } else {
+                            unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)rounded);
                         }
-                        Params unit = P;
-                        for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
-                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
                     }
                 }
             }
         }
+        if constexpr (VOL) {
+            __syncthreads();
+            if (use_lds) drain_brick(P, B, brick, acc);
+            __syncthreads();
+        }
     }
-    if constexpr (POSE) wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+    if (bad) flag(P, view, DPT_STATUS_RANGE);
+    if constexpr (POSE) {
+        if (active) {
+            const double lower_seed = -mean + span * shift_lower;
+            const double upper_seed = mean + span * shift_upper;
+            double og[3], dg[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0;
+                const double lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
+                const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0;
+                const double up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
+                og[a] = span * dor[a] + lower_seed * lo_o + upper_seed * up_o;
+                dg[a] = span * ddir[a] + lower_seed * lo_d + upper_seed * up_d;
+            }
+            World w;  // recomputed: keeps its FP64 registers out of the sample loop
+            ray_world(P, pose, view, row, col, w);
+            const double f = factor / S;
+            if (isfinite(f)) {
+                pose_epilogue<C>(P, pose, w, f, og, dg, gradient);
+            } else {
+                // kernels/projection.py:1163-1183 range-safe products
+                double sa[3], da[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double s = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+                    const double t = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+                    sa[i] = product4(seed, r.len, 1.0 / S, s);
+                    da[i] = product4(seed, r.len, 1.0 / S, t);
+                }
+                Params unit = P;
+                for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
+                pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
+            }
+        }
+        wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+    }
 }
 
 // ------------------------------------------------------- small kernels ----
@@ -1214,11 +1297,12 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
             launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
     } else {
         const double *sd = static_cast<const double *>(seeds);
+        const size_t lds = (size_t)P.brick_cap * sizeof(double);
         if (vol && pose)
-            hipLaunchKernelGGL((vjp64_kernel<T, true, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+            hipLaunchKernelGGL((vjp64_kernel<T, true, true, C>), grid, block, lds, s, P, f, sd, p->d_acc,
                                p->d_partials);
         else if (vol)
-            hipLaunchKernelGGL((vjp64_kernel<T, true, false, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+            hipLaunchKernelGGL((vjp64_kernel<T, true, false, C>), grid, block, lds, s, P, f, sd, p->d_acc,
                                p->d_partials);
         else
             hipLaunchKernelGGL((vjp64_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
