@@ -701,10 +701,17 @@ __device__ __forceinline__ double sample_cell64(const Cell64 &c, const double *v
     return value;
 }
 
+// The eight corners as FP64, fetched with the x-pair loads of the FP32 path
+// on grids with nx > 1 (uniform branch).
 template <typename T>
 __device__ __forceinline__ void load8d(const Params &P, const T *field, int base, double *v) {
+    float f[8];
+    if (P.sx)
+        load8<T, kLayoutPair>(P, field, base, f);
+    else
+        load8<T, kLayoutScalar>(P, field, base, f);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = (double)load_value<T>(field, base + corner_offset(P, k));
+    for (int k = 0; k < 8; ++k) v[k] = (double)f[k];
 }
 
 // kernels/projection.py:802-846 (midpoint, FP64 sampling)
@@ -731,14 +738,18 @@ __global__ __launch_bounds__(kBlock) void forward64_kernel(Params P, const T *__
     if (iv.upper > iv.lower) {
         const double step = (iv.upper - iv.lower) / (double)P.samples;
         bool bad = false;
+        double v[8];
+        int cur = -1;  // corners cached while consecutive samples share a cell
         for (int k = 0; k < P.samples; ++k) {
             const double pos = iv.lower + ((double)k + 0.5) * step;
             const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1], r.o[2] + pos * r.d[2]};
             bad |= !finite3d(p);
             Cell64 c;
             if (cell64(P, p, c)) {
-                double v[8];
-                load8d<T>(P, field, c.base, v);
+                if (c.base != cur) {
+                    load8d<T>(P, field, c.base, v);
+                    cur = c.base;
+                }
                 integral += sample_cell64<false>(c, v, nullptr);
             }
         }
@@ -827,6 +838,8 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
     if constexpr (VOL) {
         for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
     }
+    double v[8];
+    int cur = -1;  // corners cached while consecutive samples share a cell
     const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
     for (int ch = 0; ch < chunks; ++ch) {
         const int k0 = ch * P.chunk_samples;
@@ -861,8 +874,11 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                 Cell64 c;
                 if (!cell64(P, p, c)) continue;
                 if constexpr (POSE) {
-                    double v[8], g[3];
-                    load8d<T>(P, field, c.base, v);
+                    double g[3];
+                    if (c.base != cur) {
+                        load8d<T>(P, field, c.base, v);
+                        cur = c.base;
+                    }
                     const double value = sample_cell64<true>(c, v, g);
                     mean += value;
 #pragma unroll
