@@ -526,6 +526,44 @@ def main() -> int:
     total = time.perf_counter() - start
     assert all(torch.isfinite(g).all() for g in grads)
 
+    # the reference's component table (docs/performance/fullscan.md:248-255,
+    # 323): completed resident calls, median of 3, the same Projector
+    def timed(fn):
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[1]
+
+    module = make()
+    frozen_field, frozen_pose = field.detach(), pose.detach()
+    with torch.no_grad():
+        seed_image = module(frozen_field, frozen_pose) - target
+
+    def forward_only():
+        with torch.no_grad():
+            module(frozen_field, frozen_pose)
+
+    def volume_vjp():
+        image = module(field, frozen_pose)
+        torch.autograd.grad(image, field, seed_image)
+
+    def pose_vjp():
+        image = module(frozen_field, pose)
+        torch.autograd.grad(image, pose, seed_image)
+
+    components = {}
+    for name, fn, ref_ms in (("forward", forward_only, 512.60), ("forward_volume_vjp", volume_vjp, 1058.0),
+                             ("forward_pose_vjp", pose_vjp, None)):
+        t = timed(fn)
+        components[name] = {"ms": round(t * 1e3, 3), "gsamples_per_s": round(nominal / t / 1e9, 3)}
+        if ref_ms is not None:
+            components[name]["reference_blackwell_ms"] = ref_ms
+    _log("components: " + ", ".join(f"{k} {v['ms']} ms" for k, v in components.items()))
+
     median = float(np.median(times))
     gsamples = nominal / median / 1e9
     fwd = float(np.mean(fwd_ms))
@@ -588,6 +626,7 @@ def main() -> int:
             "iteration_frac": round(iteration_bytes / median / 1e9 / HBM_PEAK_GBS, 4),
         },
         "reference_blackwell_iteration_gsamples": 65.5,
+        "components": components,
     }
     if not args.no_serving:
         _log("serving benchmark")
