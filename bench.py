@@ -624,6 +624,8 @@ def main() -> int:
             "forward_kernel_ms": round(fwd, 3),
             "vjp_kernel_ms": round(vjp, 3),
             "iteration_frac": round(iteration_bytes / median / 1e9 / HBM_PEAK_GBS, 4),
+            "note": "algorithmic bytes over kernel time; frac > 1 means the gathers are served from L1/L2 "
+                    "(traffic is the calibrated L2-to-fabric bytes); the kernel is bound by VALU issue",
         },
         "reference_blackwell_iteration_gsamples": 65.5,
         "components": components,
