@@ -366,6 +366,51 @@ def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
 
+def gpu_parity(grid, geometry, values, samples, picks, image, seed, grad_field, grad_pose) -> dict:
+    """The HIP path (default FP32 plan, through the C ABI) on the same views,
+    field and cotangent as the CPU oracle run: relative L2 of the image and of
+    the volume and pose gradients (the correctness gate of the timed
+    workload, at its full detector size and sample count)."""
+    import numpy as np
+    import torch
+
+    from dpt_amd import _native
+
+    panels = np.asarray([geometry.views[k].packed() for k in picks], dtype=np.float64)
+    rows, cols = geometry.shape_rows_columns
+    plan = _native.Plan(grid, views=len(picks), rows=rows, cols=cols, panels=panels, rays=None, samples=samples,
+                        arithmetic="float32", storage="float32",
+                        needs=_native.VOLUME | _native.POSE | _native.GATHER_CELLS, device=0)
+    stream = torch.cuda.current_stream().cuda_stream
+    field = torch.tensor(values, device="cuda:0")
+    poses = torch.tensor([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], dtype=torch.float64,
+                         device="cuda:0").expand(len(picks), 12).contiguous()
+    out = torch.empty((len(picks), rows, cols), device="cuda:0")
+    plan.forward(field.data_ptr(), poses.data_ptr(), out.data_ptr(), stream)
+    cot = torch.tensor(np.ascontiguousarray(seed, dtype=np.float32), device="cuda:0")
+    gv = torch.empty(grid.shape, device="cuda:0")
+    gp = torch.empty((len(picks), 12), dtype=torch.float64, device="cuda:0")
+    plan.vjp(field.data_ptr(), poses.data_ptr(), cot.data_ptr(), gv.data_ptr(), gp.data_ptr(),
+             _native.VOLUME | _native.POSE, stream)
+    status = plan.status(stream)
+    plan.close()
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    gpu_image = out.cpu().numpy()
+    return {
+        "views": picks,
+        "image_rel_l2": rel(gpu_image, image),
+        "image_bitwise_equal_fraction": float(np.mean(gpu_image == image)),
+        "volume_grad_rel_l2": rel(gv.cpu().numpy(), grad_field),
+        "pose_grad_rel_l2": rel(gp.cpu().numpy(), grad_pose),
+        "status": [int(c) for c in status],
+        "bar": "1e-5 relative L2 (FP32 path)",
+    }
+
+
 def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
     """Time the CPU oracle (FP32 restatement) on views {0,90,180,270}, every
     ``stride``-th detector row, full columns and samples: forward + joint VJP,
@@ -391,14 +436,17 @@ def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
                                scene_grid.orientation), s, panels=panels)
         t0 = time.perf_counter()
         image, _ = scene.forward(scene_values)
-        scene.vjp(scene_values, seed_fn(image))
+        seed = seed_fn(image)
+        g, pg, _, _ = scene.vjp(scene_values, seed)
         elapsed = time.perf_counter() - t0
         nominal = len(picks) * panels[0].shape[0] * panels[0].shape[1] * s
+        run.last = (image, seed, g, pg)
         return nominal, elapsed, panels[0].shape
 
     residual_like = lambda image: (image - 0.5 * image.mean()).astype(np.float32)  # noqa: E731
     picks = [v for v in (0, 90, 180, 270) if v < len(geometry.views)]
     nominal, elapsed, shape = run(picks, stride, grid, values, geometry, samples, residual_like)
+    parity = gpu_parity(grid, geometry, values, samples, picks, *run.last) if stride == 1 else None
     threads = O.threads()
     O.set_threads(1)
     try:
@@ -430,6 +478,7 @@ def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
             f"{samples} samples/ray: {nominal:.3e} nominal samples in {elapsed:.2f} s; a full 360-view pass "
             f"extrapolates to {elapsed * len(geometry.views) / len(picks) * stride:.0f} s"
         ),
+        **({"parity": parity} if parity else {}),
         "single_thread": {
             "value": round(one_nominal / one_elapsed / 1e9, 6),
             "unit": "Gsamples/s",
