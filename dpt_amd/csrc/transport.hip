@@ -321,27 +321,51 @@ __device__ __forceinline__ double form_factor(const Model &m, const double *T, d
     return (1.0 - fraction) * fs[lo] + fraction * fs[hi];
 }
 
-// physical_kernels.py:99-127.
-__device__ __forceinline__ Angular rayleigh_event(const Model &m, const double *T, double energy, int material,
-                                                  uint64_t seed, uint64_t history, uint32_t event) {
+// physical_kernels.py:99-127: rejection with c = 2U - 1, q = E/hc sqrt((1-c)/2),
+// accept U' < (1 + c^2)/2 (F(q)/Z)^2; trial t draws from domain 0xC0000000 + t.
+// Acceptance is ~0.6 % at 60 keV (~170 trials per event), so one lane's loop
+// would hold its whole wave for hundreds of iterations.  The wave evaluates
+// 64 consecutive trials of one photon at a time instead (lane l takes trial
+// base + l) and keeps the first accepted one: the same trial, and the same
+// cosine and azimuth, as the sequential loop.  Called with every lane of the
+// wave active; every argument is uniform.
+__device__ __forceinline__ Angular rayleigh_event_wave(const Model &m, const double *T, double energy, int material,
+                                                       uint64_t seed, uint64_t history, uint32_t event, int lane) {
     Angular r{0.0, 0.0, energy, ANGLE_BUDGET};
     const double z = T[m.off_z + material];
-    for (int trial = 0; trial < m.max_trials; ++trial) {
-        double u[4];
-        random4(seed, history, event, 0xC0000000u + (uint32_t)trial, u);
-        const double cosine = 2.0 * u[0] - 1.0;
-        const double q = energy / kHcKeVAngstrom * sqrt((1.0 - cosine) * 0.5);
-        const double factor = form_factor(m, T, q, material) / z;
-        const double acceptance = 0.5 * (1.0 + cosine * cosine) * factor * factor;
-        if (u[1] < acceptance) {
-            r.cosine = cosine;
-            r.azimuth = 2.0 * kPi * u[2];
+    for (int base = 0; base < m.max_trials; base += 64) {
+        const int trial = base + lane;
+        bool accept = false;
+        double cosine = 0.0, azimuth = 0.0;
+        if (trial < m.max_trials) {
+            double u[4];
+            random4(seed, history, event, 0xC0000000u + (uint32_t)trial, u);
+            cosine = 2.0 * u[0] - 1.0;
+            const double q = energy / kHcKeVAngstrom * sqrt((1.0 - cosine) * 0.5);
+            const double factor = form_factor(m, T, q, material) / z;
+            const double acceptance = 0.5 * (1.0 + cosine * cosine) * factor * factor;
+            accept = u[1] < acceptance;
+            azimuth = 2.0 * kPi * u[2];
+        }
+        const uint64_t hits = __ballot(accept);
+        if (hits != 0ull) {
+            const int first = __ffsll((unsigned long long)hits) - 1;
+            r.cosine = __shfl(cosine, first, 64);
+            r.azimuth = __shfl(azimuth, first, 64);
             r.status = ESCAPED;
             break;
         }
     }
     return r;
 }
+
+// A Rayleigh event left for the wave to sample (event_step returns before
+// the angular draw; the photon's remaining work is finish_rayleigh).
+struct PendingRayleigh {
+    bool pending;
+    int material;
+    uint32_t event;
+};
 
 // State of the photon a lane carries.
 struct Photon {
@@ -434,7 +458,8 @@ __device__ __forceinline__ void start(const Model &m, const Call &c, const doubl
 }
 
 // One iteration of the reference's `while alive` event loop.
-__device__ __forceinline__ void event_step(const Model &m, const Call &c, const double *T, Photon &ph) {
+__device__ __forceinline__ void event_step(const Model &m, const Call &c, const double *T, Photon &ph,
+                                           PendingRayleigh &pend) {
     if (m.physics && ph.energy <= m.cutoff) {
         ph.energy = 0.0;
         ph.status = ABSORBED;
@@ -620,14 +645,11 @@ __device__ __forceinline__ void event_step(const Model &m, const Call &c, const 
             ph.status = NUMERICAL_FAILURE;
             ph.alive = false;
         } else if (channel[0] < rayleigh_rate / s_rate) {
-            coherent = true;
-            const Angular r = rayleigh_event(m, T, ph.energy, material, c.seed, ph.history, event);
-            cosine = r.cosine;
-            azimuth = r.azimuth;
-            if (r.status != ESCAPED) {
-                ph.status = ANGLE_BUDGET;
-                ph.alive = false;
-            }
+            // sampled by the whole wave (rayleigh_event_wave), then finish_rayleigh
+            pend.pending = true;
+            pend.material = material;
+            pend.event = event;
+            return;
         }
     }
     if (ph.alive && m.compton && !coherent) {
@@ -642,6 +664,16 @@ __device__ __forceinline__ void event_step(const Model &m, const Call &c, const 
         }
     }
     if (ph.alive) scatter_direction(ph.dir, cosine, azimuth);
+}
+
+// The rest of a Rayleigh event once its angles are drawn.
+__device__ __forceinline__ void finish_rayleigh(Photon &ph, const Angular &r) {
+    if (r.status != ESCAPED) {
+        ph.status = ANGLE_BUDGET;
+        ph.alive = false;
+        return;
+    }
+    scatter_direction(ph.dir, r.cosine, r.azimuth);
 }
 
 __global__ void zero_counter_kernel(unsigned long long *counter) { *counter = 0ull; }
@@ -688,8 +720,18 @@ __global__ __launch_bounds__(kBlock, WAVES) void trace_kernel(Model m, Call c) {
             need = __ballot(!has);
         }
         if (__ballot(has) == 0ull) break;
+        PendingRayleigh pend{false, 0, 0u};
+        if (has && ph.alive) event_step(m, c, T, ph, pend);
+        // Rayleigh draws, one photon at a time by the whole wave (all lanes
+        // active here; the loop bound and the leader's values are uniform)
+        for (uint64_t waiting = __ballot(pend.pending); waiting != 0ull; waiting &= waiting - 1ull) {
+            const int leader = __ffsll((unsigned long long)waiting) - 1;
+            const Angular r = rayleigh_event_wave(m, T, __shfl(ph.energy, leader, 64), __shfl(pend.material, leader, 64),
+                                                  c.seed, __shfl(ph.history, leader, 64),
+                                                  (uint32_t)__shfl((int)pend.event, leader, 64), lane);
+            if (lane == leader) finish_rayleigh(ph, r);
+        }
         if (has) {
-            if (ph.alive) event_step(m, c, T, ph);
             if (!ph.alive) {
                 // Termination: detector hit and score (transport/kernels.py:1520-1536).
                 int pixel = kMissPixel;
