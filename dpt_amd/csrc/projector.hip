@@ -12,13 +12,17 @@
 // is written out, so the per-ray results are a fixed function of the inputs.
 //
 // Execution model (not the reference's): one launch covers all views.  A
-// workgroup of 256 lanes owns one 16x16 detector tile of one view (four wave64
-// quadrants of 8x8 rays); workgroup ids are remapped so each XCD walks a
-// contiguous range of tiles.  Pose cotangents are reduced per ray in FP64,
-// across the wave with shuffles and across the tile in LDS, then across tiles
-// in a fixed-order FP64 pass.  Volume cotangents are summed per cell in
-// registers (FP32 increments, reference flush-to-zero rule) and added to an
-// FP64 accumulator with global atomics; a final pass rounds to FP32.
+// workgroup of 256 lanes owns one 16x16 detector tile of one view, or 8x8
+// tiles of four consecutive views (one wave64 each); workgroup ids are
+// remapped so each XCD walks a contiguous range of tiles.  Corners are
+// gathered from the linear field (x-adjacent pairs) or from a cell-layout copy
+// (8 corners per cell, one or two 16-byte loads).  Pose cotangents are
+// reduced per ray in FP64, across the wave with shuffles, then across tiles in
+// a fixed-order FP64 pass.  Volume cotangents are summed per cell in
+// registers (FP32 increments, reference flush-to-zero rule), added per chunk
+// of samples into an FP64 LDS brick of the cells the workgroup touches, and
+// drained into an FP64 accumulator with one device atomic per touched grid
+// sample; a final pass rounds to FP32.
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         if (active) {
             // Lockstep walk of this chunk with the current cell cached (see
             // forward32_kernel); cell changes trigger masked loads and the
-            // shared-face transition of the register sums.
+            // transition of the register sums.
             bad |= !finite_span(o, d, iv.lower, step, k0, k1);
             float v[8];
             int cur = -1;
