@@ -162,6 +162,33 @@ def test_fp32_vjp_matches_oracle(name, mode, components):
         np.testing.assert_allclose(p_gpu, p_ref, rtol=1e-4, atol=1e-6 * np.abs(p_ref).max())
 
 
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("cap,chunk", [("512", None), ("512", "7"), (None, "1"), (None, "64"), ("8192", "16")])
+def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch):
+    """LDS bricks of every size down to the smallest capacity (512 entries),
+    where most chunks overflow and take the device-atomic path with the
+    shared-face carry; chunk lengths from 1 to 64 samples."""
+    print(f"case bricks cap={cap} chunk={chunk} {precision}", flush=True)
+    if cap is not None:
+        monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
+    if chunk is not None:
+        monkeypatch.setenv("DPT_AMD_CHUNK", chunk)
+    grid, panels, values = _cone(n=24, views=5, rows=20, cols=28, seed=4)
+    pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    ref = O.Scene(grid, 97, panels=panels)
+    gpu = NativeScene(grid, 97, panels=panels, precision=precision)
+    fp = "fp32" if precision == "float32" else "fp64"
+    image, _ = ref.forward(values, pose, precision=fp)
+    seeds = np.random.default_rng(8).uniform(-1.0, 1.0, image.size)
+    if precision == "float32":
+        seeds = seeds.astype(np.float32)
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose, precision=fp)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose)
+    assert status.tolist() == [0] * len(status)
+    assert _rel(g_gpu, g_ref) <= (1e-5 if precision == "float32" else 1e-7)
+    assert _rel(p_gpu, p_ref) <= (1e-5 if precision == "float32" else 1e-10)
+
+
 @pytest.mark.parametrize("mode", ["both", "volume", "pose"])
 @pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
 def test_fp64_vjp_matches_oracle(name, mode):
