@@ -642,6 +642,7 @@ def main() -> int:
         "ms_per_step": round(total / args.steps * 1e3, 3),
         "median_ms": round(median * 1e3, 3),
         "p95_ms": round(float(np.percentile(times, 95)) * 1e3, 3),
+        "step_ms": [round(t * 1e3, 3) for t in times],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
