@@ -23,6 +23,7 @@ EXPORTS = (
     "dpt_plan_destroy",
     "dpt_forward",
     "dpt_vjp",
+    "dpt_adjoint_gather",
     "dpt_status",
     "dpt_convert_field",
     "dpt_cell_count",
@@ -174,6 +175,8 @@ def lib():
         h.dpt_forward.restype = ctypes.c_int
         h.dpt_vjp.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp]
         h.dpt_vjp.restype = ctypes.c_int
+        h.dpt_adjoint_gather.argtypes = [vp, vp, vp, vp, i32, u32, vp]
+        h.dpt_adjoint_gather.restype = ctypes.c_int
         h.dpt_status.argtypes = [vp, vp, i32, vp]
         h.dpt_status.restype = ctypes.c_int
         h.dpt_convert_field.argtypes = [vp, vp, i64, i32, vp, vp]
@@ -276,6 +279,13 @@ class Plan:
         check(
             lib().dpt_vjp(self.handle, field, poses, cotangent, grad_field, grad_pose, flags, stream),
             "dpt_vjp",
+        )
+
+    def adjoint_gather(self, poses: int, cotangent: int, out: int, out_precision: str, flags: int,
+                       stream: int) -> None:
+        check(
+            lib().dpt_adjoint_gather(self.handle, poses, cotangent, out, ARITH_CODES[out_precision], flags, stream),
+            "dpt_adjoint_gather",
         )
 
     def status(self, stream: int, clear: bool = True) -> np.ndarray:

@@ -23,7 +23,12 @@ import sys
 from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
-SOURCES = [PACKAGE / "csrc" / "projector.hip", PACKAGE / "csrc" / "fit.hip", PACKAGE / "csrc" / "transport.hip"]
+SOURCES = [
+    PACKAGE / "csrc" / "projector.hip",
+    PACKAGE / "csrc" / "gather.hip",
+    PACKAGE / "csrc" / "fit.hip",
+    PACKAGE / "csrc" / "transport.hip",
+]
 OUTPUT = PACKAGE / "_lib" / "libdpt_amd.so"
 ARCH = os.environ.get("DPT_AMD_ARCH", "gfx950")
 

@@ -193,6 +193,92 @@ __device__ __forceinline__ void ray_grid(const Params &P, const World &w, Ray &r
               finite3d(w.wsr) && finite3d(w.wd) && isfinite(r.len) && r.len > 0.0;
 }
 
+// --------------------------------------------------------- FP64 helpers --
+
+struct Interval64 {
+    double lower, upper;
+    int lo_axis, up_axis;
+    double lo_o, lo_d, up_o, up_d;
+};
+
+// kernels/projection.py:147-184
+__device__ __forceinline__ Interval64 interval64(const Params &P, const double *o, const double *d) {
+    Interval64 iv;
+    iv.lower = 0.0;
+    iv.upper = 1.0;
+    iv.lo_axis = iv.up_axis = -1;
+    iv.lo_o = iv.lo_d = iv.up_o = iv.up_d = 0.0;
+    const double highs[3] = {(double)P.nx - 0.5, (double)P.ny - 0.5, (double)P.nz - 0.5};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double low = -0.5, high = highs[a], v = d[a];
+        if (v == 0.0) {
+            if (o[a] < low || o[a] > high) iv.upper = -1.0;
+        } else {
+            double first = (low - o[a]) / v, last = (high - o[a]) / v;
+            if (first > last) {
+                const double t = first;
+                first = last;
+                last = t;
+            }
+            if (first > iv.lower) {
+                iv.lower = first;
+                iv.lo_axis = a;
+                iv.lo_o = -1.0 / v;
+                iv.lo_d = -first / v;
+            }
+            if (last < iv.upper) {
+                iv.upper = last;
+                iv.up_axis = a;
+                iv.up_o = -1.0 / v;
+                iv.up_d = -last / v;
+            }
+        }
+    }
+    return iv;
+}
+
+// kernels/numerics.py:11-56: product through a mantissa/exponent split
+__device__ __forceinline__ double product4(double a, double b, double c, double d) {
+    const double f[4] = {a, b, c, d};
+    double ordinary = a * b * c * d;
+    bool invalid = false, zero = false;
+    for (int i = 0; i < 4; ++i) {
+        invalid |= !isfinite(f[i]);
+        zero |= f[i] == 0.0;
+    }
+    if (invalid) return ordinary;
+    if (zero) return 0.0;
+    double m = 1.0;
+    int e = 0;
+    for (int i = 0; i < 4; ++i) {
+        int ei;
+        const double mi = frexp(f[i], &ei);
+        m = (i == 0) ? mi : m * mi;
+        e += ei;
+    }
+    return ldexp(m, e);
+}
+
+
+// kernels/batched_geometry.py:71-88 + kernels/geometry.py:55-74: finite
+// entries, orthonormal to 1e-10 and a proper rotation.
+__device__ __forceinline__ bool pose_is_rigid(const double *R) {
+    bool valid = isfinite(R[9]) && isfinite(R[10]) && isfinite(R[11]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            if (!isfinite(R[3 * i + j])) valid = false;
+            // (R^T R)[i][j] = sum_k R[k][i] R[k][j]
+            const double prod = R[i] * R[j] + R[3 + i] * R[3 + j] + R[6 + i] * R[6 + j];
+            const double expected = (i == j) ? 1.0 : 0.0;
+            if (!(fabs(prod - expected) <= 1e-10)) valid = false;
+        }
+    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                       R[2] * (R[3] * R[7] - R[4] * R[6]);
+    if (det < 0.0) valid = false;
+    return valid;
+}
+
 // --------------------------------------------------------- FP32 sampling --
 
 struct Interval32 {

@@ -1,0 +1,40 @@
+// Deterministic adjoint gather: launch interface shared by gather.hip (the
+// kernels) and projector.hip (the plan-owning C entry point).
+#pragma once
+
+#include "device_common.h"
+
+namespace dpt {
+
+// One ray of the gather, prepared once per call: the FP64 ray of the forward
+// march (kernels/projection.py:79-121) with its clipped interval, sample
+// step and volume factor (kernels/projection_adjoint.py:94-157), plus the
+// reciprocals the culling uses.  vf == 0 marks a ray that adds nothing
+// (invalid, zero cotangent or empty interval).
+struct alignas(16) GatherRay {
+    double o[3], d[3], inv[3];  // grid-frame origin, direction, 1/direction (0 where direction is 0)
+    double lower, step, inv_step, vf, pad;
+};
+
+// Per-view projection of grid points onto the panel's (column, row)
+// coordinates for the detector culling: a point x lies on the ray of pixel
+// (c, r) when (x - s).B = c (x - s).A and (x - s).C = r (x - s).A, with
+// A = U x V, B = V x D0, C = D0 x U in grid coordinates (U, V: column and
+// row steps, D0: direction of pixel (0, 0), s: source), signs chosen so that
+// (x - s).A > 0 in front of the source.
+struct alignas(16) GatherView {
+    double s[3], a[3], b[3], c[3];
+    double norm_a;  // |A|
+    double ratio;   // (|B| + |C|) / |A|, the scale of the pixel-coordinate rounding
+    double margin;  // support-box widening for this view's rays, grid units
+    int ok;         // 0: cull nothing (explicit rays or a degenerate panel)
+    int pad;
+};
+
+// Enqueue the gather: guard reset, cotangent/pose checks (checks != 0), view
+// and ray preparation, then one thread per voxel.  out is double (out_fp64)
+// or float; the guard word suppresses every write when a check failed.
+hipError_t launch_gather(const Params &P, const double *cotangent, GatherRay *rays, GatherView *views, int *guard,
+                         void *out, bool out_fp64, bool accumulate, bool checks, hipStream_t stream);
+
+}  // namespace dpt

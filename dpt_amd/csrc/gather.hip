@@ -1,0 +1,415 @@
+// Deterministic adjoint gather for gfx950: out[v] = sum_p A[p, v] c[p] in
+// FP64 with one thread per voxel, so every voxel's sum has a fixed order and
+// the result is the same bit for bit on every run (the ray-driven volume
+// adjoint accumulates with atomics and is not).
+//
+// Restates kernels/projection_adjoint.py:73-170 (get_gather, midpoint):
+// for each voxel, views in order, pixels row-major, samples in order and the
+// 8 cell corners in order, add scaled_corner_weight(fraction, corner,
+// volume_factor) (kernels/projection.py:396-404, ((vf * w_x) * w_y) * w_z,
+// no FP32 rounding) for every corner whose clamped index is the voxel.
+// Compiled with -ffp-contract=off and written in the reference's operation
+// order, so the sums equal the CPU restatement's bit for bit.
+//
+// Culling (the role of kernels/projection_adjoint.py:330-597, designed
+// here independently): a voxel can only be a corner of a sample whose
+// clamped cell coordinates lie in [i-1, i+1] per axis (to the support
+// boundary -0.5 / n-0.5 at the grid edges).  The kernel visits only the
+// pixels whose rays can reach that box, found by projecting the box's 8
+// corners onto the panel from the source, and only the samples of each ray
+// whose points can fall in it, found by slab clipping.  Both enclosures are
+// widened by margins many orders of magnitude above the FP64 rounding of
+// the sample points, so every skipped sample adds exactly nothing (no corner
+// of its cell is the voxel) and the sums are those of the full loop.  A view
+// whose panel is nearly edge-on to part of the box falls back to all pixels.
+#include "gather.h"
+
+#include <cmath>
+
+namespace dpt {
+
+namespace {
+
+// Relative support-box widening (grid units per unit of coordinate
+// magnitude): the rounding of a sample point is below 1e-15 of the
+// magnitudes involved.
+constexpr double kBoxMargin = 1e-9;
+// Relative widening of a projected pixel range and of a ray parameter range.
+constexpr double kPixelMargin = 1e-9;
+constexpr double kParamMargin = 1e-12;
+// Minimum (x - s).A / (|x - s| |A|) at every box corner for the projection
+// to be used (about 0.06 degrees off the plane through the source parallel
+// to the panel); otherwise the view visits every pixel.
+constexpr double kFacing = 1e-3;
+// Steps below this (grid units) visit every sample of the ray.
+constexpr double kMinStep = 1e-12;
+
+__device__ __forceinline__ void grid_vector(const Params &P, const double *pose, const double *x, double *out) {
+    double od[3];
+    for (int i = 0; i < 3; ++i) od[i] = pose[i] * x[0] + pose[3 + i] * x[1] + pose[6 + i] * x[2];
+    for (int i = 0; i < 3; ++i) out[i] = P.g[3 * i] * od[0] + P.g[3 * i + 1] * od[1] + P.g[3 * i + 2] * od[2];
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *out) {
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ double max_abs3(const double *a) {
+    return fmax(fabs(a[0]), fmax(fabs(a[1]), fabs(a[2])));
+}
+
+__global__ void clear_guard_kernel(int *guard) { *guard = 0; }
+
+// Cotangent finite (NumericalError) and poses rigid (ContractError), as the
+// reference validates before launching (python/dpt/projection.py:1337-1363);
+// a failure also raises the guard so that the gather writes nothing.
+__global__ __launch_bounds__(kBlock) void gather_checks_kernel(const double *__restrict__ poses, int views,
+                                                               const double *__restrict__ cotangent, int64_t pixels,
+                                                               int *status, int *guard) {
+    const int64_t first = (int64_t)blockIdx.x * kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+    bool bad = false;
+    for (int64_t i = first; i < pixels; i += stride) bad |= !isfinite(cotangent[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) {
+        atomicOr(&status[0], DPT_STATUS_RANGE);
+        atomicOr(guard, 1);
+    }
+    for (int64_t v = first; v < views; v += stride)
+        if (!pose_is_rigid(poses + 12 * v)) {
+            atomicOr(&status[v], DPT_STATUS_POSE);
+            atomicOr(guard, 1);
+        }
+}
+
+// One thread per view: the panel projection used by the detector culling.
+__global__ void gather_views_kernel(Params P, GatherView *__restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.views) return;
+    GatherView gv;
+    for (int i = 0; i < 3; ++i) gv.s[i] = gv.a[i] = gv.b[i] = gv.c[i] = 0.0;
+    gv.norm_a = gv.ratio = gv.margin = 0.0;
+    gv.ok = 0;
+    gv.pad = 0;
+    if (!P.bundle) {
+        const double *pose = P.poses + 12 * (size_t)v;
+        const double *pn = P.panels + 12 * (size_t)v;
+        World w;
+        ray_world(P, pose, v, 0, 0, w);
+        Ray r;
+        ray_grid(P, w, r);
+        double u[3], vv[3], a[3], b[3], c[3];
+        grid_vector(P, pose, pn + 6, u);
+        grid_vector(P, pose, pn + 9, vv);
+        cross3(u, vv, a);
+        cross3(vv, r.d, b);
+        cross3(r.d, u, c);
+        const double det = dot3(r.d, a);
+        if (r.valid && isfinite(det) && det != 0.0) {
+            const double sign = det > 0.0 ? 1.0 : -1.0;
+            for (int i = 0; i < 3; ++i) {
+                gv.s[i] = r.o[i];
+                gv.a[i] = sign * a[i];
+                gv.b[i] = sign * b[i];
+                gv.c[i] = sign * c[i];
+            }
+            gv.norm_a = sqrt(dot3(a, a));
+            gv.ratio = (sqrt(dot3(b, b)) + sqrt(dot3(c, c))) / gv.norm_a;
+            const double reach = max_abs3(r.d) + (double)(P.cols - 1) * max_abs3(u) + (double)(P.rows - 1) * max_abs3(vv);
+            gv.margin = kBoxMargin * (1.0 + max_abs3(r.o) + reach);
+            gv.ok = isfinite(gv.ratio) && isfinite(gv.margin) && gv.norm_a > 0.0 &&
+                    isfinite(dot3(gv.b, gv.b)) && isfinite(dot3(gv.c, gv.c));
+        }
+    }
+    out[v] = gv;
+}
+
+// One thread per ray: setup, validity, interval and volume factor
+// (kernels/projection_adjoint.py:89-157); rays flagged and skipped as there.
+__global__ __launch_bounds__(kBlock) void gather_rays_kernel(Params P, const double *__restrict__ cotangent,
+                                                             GatherRay *__restrict__ rays) {
+    const int64_t pixels = (int64_t)P.rows * P.cols, total = (int64_t)P.views * pixels;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+        const int view = (int)(i / pixels);
+        const int pixel = (int)(i - (int64_t)view * pixels);
+        const int row = pixel / P.cols, col = pixel - row * P.cols;
+        const double *pose = P.poses + 12 * (size_t)view;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        GatherRay g;
+        for (int a = 0; a < 3; ++a) g.o[a] = g.d[a] = g.inv[a] = 0.0;
+        g.lower = g.step = g.inv_step = g.vf = g.pad = 0.0;
+        if (!r.valid) {
+            flag(P, view, DPT_STATUS_RANGE);
+        } else {
+            const double seed = cotangent[i];
+            const Interval64 iv = interval64(P, r.o, r.d);
+            const double span = iv.upper - iv.lower;
+            if (seed != 0.0 && span > 0.0) {
+                const double factor = seed * r.len;
+                const double step = span / (double)P.samples;
+                double vf = factor * span / (double)P.samples;
+                if (!isfinite(factor)) vf = product4(seed, r.len, span, 1.0 / (double)P.samples);
+                // |pos| <= 1, so points are finite unless the ray's
+                // coordinates approach the FP64 range: only then test them.
+                if (fmax(max_abs3(r.o), max_abs3(r.d)) > 1e300) {
+                    bool bad = false;
+                    for (int k = 0; k < P.samples; ++k) {
+                        const double pos = iv.lower + ((double)k + 0.5) * step;
+                        const double p[3] = {r.o[0] + pos * r.d[0], r.o[1] + pos * r.d[1], r.o[2] + pos * r.d[2]};
+                        bad |= !finite3d(p);
+                    }
+                    if (bad) flag(P, view, DPT_STATUS_RANGE);
+                }
+                for (int a = 0; a < 3; ++a) {
+                    g.o[a] = r.o[a];
+                    g.d[a] = r.d[a];
+                    g.inv[a] = r.d[a] != 0.0 ? 1.0 / r.d[a] : 0.0;
+                }
+                g.lower = iv.lower;
+                g.step = step;
+                g.inv_step = 1.0 / step;
+                g.vf = vf;
+            }
+        }
+        rays[i] = g;
+    }
+}
+
+struct Voxel {
+    int i[3];
+    double lo[3], hi[3];  // clamped-cell support: lower cell coordinate in [i-1, i] per axis
+};
+
+// Pixel window [c0, c1] x [r0, r1] of one view whose rays can reach the
+// voxel's support box; false when the projection cannot be trusted (the
+// caller then visits every pixel).
+__device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &gv, const Voxel &vx, int &c0,
+                                             int &c1, int &r0, int &r1) {
+    double cmin = INFINITY, cmax = -INFINITY, rmin = INFINITY, rmax = -INFINITY;
+    bool good = true;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        double w[3];
+        for (int a = 0; a < 3; ++a) w[a] = (((q >> a) & 1) ? vx.hi[a] + gv.margin : vx.lo[a] - gv.margin) - gv.s[a];
+        const double den = dot3(w, gv.a);
+        good = good && den > kFacing * sqrt(dot3(w, w)) * gv.norm_a;
+        const double inv = 1.0 / den;
+        const double c = dot3(w, gv.b) * inv, r = dot3(w, gv.c) * inv;
+        cmin = fmin(cmin, c);
+        cmax = fmax(cmax, c);
+        rmin = fmin(rmin, r);
+        rmax = fmax(rmax, r);
+    }
+    if (!good || !isfinite(cmin) || !isfinite(cmax) || !isfinite(rmin) || !isfinite(rmax)) return false;
+    const double mc = kPixelMargin * (1.0 + fmax(fabs(cmin), fabs(cmax)) + gv.ratio);
+    const double mr = kPixelMargin * (1.0 + fmax(fabs(rmin), fabs(rmax)) + gv.ratio);
+    // clamped to [-1, n] before the integer conversion
+    c0 = (int)ceil(fmin(fmax(cmin - mc, -1.0), (double)P.cols));
+    c1 = (int)floor(fmax(fmin(cmax + mc, (double)P.cols), -1.0));
+    r0 = (int)ceil(fmin(fmax(rmin - mr, -1.0), (double)P.rows));
+    r1 = (int)floor(fmax(fmin(rmax + mr, (double)P.rows), -1.0));
+    c0 = max(c0, 0);
+    r0 = max(r0, 0);
+    c1 = min(c1, P.cols - 1);
+    r1 = min(r1, P.rows - 1);
+    return true;
+}
+
+// Sample indices [k0, k1] of ray g whose points can lie in the voxel's
+// support box; false when none can.
+__device__ __forceinline__ bool sample_window(const Params &P, const GatherRay &g, const Voxel &vx, int &k0,
+                                              int &k1) {
+    k0 = 0;
+    k1 = P.samples - 1;
+    double tlo = -INFINITY, thi = INFINITY;
+    bool full = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double m = kBoxMargin * (1.0 + fabs(g.o[a]) + fabs(g.d[a]));
+        const double lo = vx.lo[a] - m, hi = vx.hi[a] + m;
+        if (g.d[a] == 0.0) {
+            if (g.o[a] < lo || g.o[a] > hi) return false;
+        } else {
+            double t1 = (lo - g.o[a]) * g.inv[a], t2 = (hi - g.o[a]) * g.inv[a];
+            if (isnan(t1) || isnan(t2)) {
+                full = true;
+            } else {
+                if (t1 > t2) {
+                    const double t = t1;
+                    t1 = t2;
+                    t2 = t;
+                }
+                tlo = fmax(tlo, t1);
+                thi = fmin(thi, t2);
+            }
+        }
+    }
+    if (full) return true;
+    if (tlo > thi) return false;
+    tlo -= kParamMargin * (1.0 + fabs(tlo));
+    thi += kParamMargin * (1.0 + fabs(thi));
+    if (!(g.step > kMinStep)) return true;
+    double kf0 = (tlo - g.lower) * g.inv_step - 0.5, kf1 = (thi - g.lower) * g.inv_step - 0.5;
+    if (isnan(kf0) || isnan(kf1)) return true;
+    kf0 = fmin(fmax(kf0, -2.0), (double)P.samples + 1.0);
+    kf1 = fmax(fmin(kf1, (double)P.samples + 1.0), -2.0);
+    if (kf0 > kf1) return false;
+    k0 = max(0, (int)floor(kf0) - 1);
+    k1 = min(P.samples - 1, (int)ceil(kf1) + 1);
+    return k0 <= k1;
+}
+
+// Add the contributions of ray g's samples k0..k1 to the voxel's total, in
+// sample then corner order (kernels/projection_adjoint.py:158-163 with
+// point_contribution, :33-44).  SIMPLE: every axis has >= 2 samples, so at
+// most one corner of a cell is the voxel.
+template <bool SIMPLE>
+__device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, const Voxel &vx, int k0, int k1,
+                                          double total) {
+    const int n[3] = {P.nx, P.ny, P.nz};
+    const int cap[3] = {P.capx, P.capy, P.capz};
+    const double vf = g.vf;
+    for (int k = k0; k <= k1; ++k) {
+        const double pos = g.lower + ((double)k + 0.5) * g.step;
+        double p[3];
+        for (int a = 0; a < 3; ++a) p[a] = g.o[a] + pos * g.d[a];
+        // kernels/projection.py:124-134 (inside_support; NaN fails every test)
+        if (!(p[0] >= -0.5 && p[1] >= -0.5 && p[2] >= -0.5 && p[0] <= (double)n[0] - 0.5 &&
+              p[1] <= (double)n[1] - 0.5 && p[2] <= (double)n[2] - 0.5))
+            continue;
+        int l[3];
+        double f[3];
+        // kernels/projection.py:199-214 (cell_lower, cell_fraction)
+        for (int a = 0; a < 3; ++a) {
+            const double c = fmin(fmax(p[a], 0.0), (double)(n[a] - 1));
+            l[a] = min((int)floor(c), cap[a]);
+            f[a] = c - (double)l[a];
+        }
+        if constexpr (SIMPLE) {
+            bool hit = true;
+            double w[3];
+            for (int a = 0; a < 3; ++a) {
+                hit = hit && (l[a] == vx.i[a] || l[a] + 1 == vx.i[a]);
+                w[a] = l[a] == vx.i[a] ? 1.0 - f[a] : f[a];
+            }
+            if (hit) total += ((vf * w[0]) * w[1]) * w[2];
+        } else {
+            bool m0[3], m1[3];
+            for (int a = 0; a < 3; ++a) {
+                m0[a] = l[a] == vx.i[a];
+                m1[a] = min(l[a] + 1, n[a] - 1) == vx.i[a];
+            }
+#pragma unroll
+            for (int corner = 0; corner < 8; ++corner) {
+                const int bx = corner & 1, by = (corner >> 1) & 1, bz = (corner >> 2) & 1;
+                if ((bx ? m1[0] : m0[0]) && (by ? m1[1] : m0[1]) && (bz ? m1[2] : m0[2]))
+                    total += ((vf * (bx ? f[0] : 1.0 - f[0])) * (by ? f[1] : 1.0 - f[1])) * (bz ? f[2] : 1.0 - f[2]);
+            }
+        }
+    }
+    return total;
+}
+
+// Voxel bricks of 8 x 8 x 4 per workgroup (4 x 4 x 4 per wave, so a wave's
+// voxels see nearly the same rays); consecutive workgroups go to different
+// XCDs, so each XCD takes a contiguous range of bricks and its L2 keeps the
+// ray records of neighbouring bricks.
+template <typename OUT, bool SIMPLE>
+__global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRay *__restrict__ rays,
+                                                        const GatherView *__restrict__ views,
+                                                        const int *__restrict__ guard, OUT *__restrict__ out,
+                                                        int accumulate, int bricks_x, int bricks_y, int bricks,
+                                                        int chunk) {
+    if (*guard) return;  // a failed check: the destination stays untouched
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * chunk + b / kXcds;
+    if (logical >= bricks) return;
+    const int bx = logical % bricks_x, rest = logical / bricks_x;
+    const int by = rest % bricks_y, bz = rest / bricks_y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    Voxel vx;
+    vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
+    vx.i[1] = by * 8 + (w >> 1) * 4 + ((l >> 2) & 3);
+    vx.i[2] = bz * 4 + (l >> 4);
+    const int n[3] = {P.nx, P.ny, P.nz};
+    if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;  // no barriers below
+    for (int a = 0; a < 3; ++a) {
+        vx.lo[a] = vx.i[a] <= 1 ? -0.5 : (double)(vx.i[a] - 1);
+        vx.hi[a] = vx.i[a] >= n[a] - 2 ? (double)n[a] - 0.5 : (double)(vx.i[a] + 1);
+    }
+    const int64_t pixels = (int64_t)P.rows * P.cols;
+    double total = 0.0;
+    for (int view = 0; view < P.views; ++view) {
+        int c0 = 0, c1 = P.cols - 1, r0 = 0, r1 = P.rows - 1;
+        if (!P.bundle) {
+            const GatherView gv = views[view];
+            if (gv.ok && !pixel_window(P, gv, vx, c0, c1, r0, r1)) {
+                c0 = 0;
+                c1 = P.cols - 1;
+                r0 = 0;
+                r1 = P.rows - 1;
+            }
+        }
+        const GatherRay *base = rays + (int64_t)view * pixels;
+        for (int row = r0; row <= r1; ++row)
+            for (int col = c0; col <= c1; ++col) {
+                const GatherRay &g = base[(int64_t)row * P.cols + col];
+                if (g.vf == 0.0) continue;
+                int k0, k1;
+                if (!sample_window(P, g, vx, k0, k1)) continue;
+                total = add_ray<SIMPLE>(P, g, vx, k0, k1, total);
+            }
+    }
+    const int64_t voxel = ((int64_t)vx.i[2] * n[1] + vx.i[1]) * n[0] + vx.i[0];
+    if (accumulate) total += (double)out[voxel];
+    const OUT value = (OUT)total;
+    out[voxel] = value;
+    if (!isfinite((double)value)) atomicOr(&P.status[0], DPT_STATUS_RANGE);
+}
+
+unsigned stride_blocks(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+template <typename OUT>
+void launch_voxels(const Params &P, const GatherRay *rays, const GatherView *views, const int *guard, OUT *out,
+                   int accumulate, hipStream_t s) {
+    const int bricks_x = (P.nx + 7) / 8, bricks_y = (P.ny + 7) / 8, bricks_z = (P.nz + 3) / 4;
+    const int bricks = bricks_x * bricks_y * bricks_z;
+    const int chunk = (bricks + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(chunk * kXcds)), block(kBlock);
+    if (P.nx > 1 && P.ny > 1 && P.nz > 1)
+        hipLaunchKernelGGL((gather_kernel<OUT, true>), grid, block, 0, s, P, rays, views, guard, out, accumulate,
+                           bricks_x, bricks_y, bricks, chunk);
+    else
+        hipLaunchKernelGGL((gather_kernel<OUT, false>), grid, block, 0, s, P, rays, views, guard, out, accumulate,
+                           bricks_x, bricks_y, bricks, chunk);
+}
+
+}  // namespace
+
+hipError_t launch_gather(const Params &P, const double *cotangent, GatherRay *rays, GatherView *views, int *guard,
+                         void *out, bool out_fp64, bool accumulate, bool checks, hipStream_t s) {
+    const int64_t pixels = (int64_t)P.views * P.rows * P.cols;
+    hipLaunchKernelGGL(clear_guard_kernel, dim3(1), dim3(1), 0, s, guard);
+    if (checks)
+        hipLaunchKernelGGL(gather_checks_kernel, dim3(stride_blocks(pixels > P.views ? pixels : P.views)),
+                           dim3(kBlock), 0, s, P.poses, P.views, cotangent, pixels, P.status, guard);
+    hipLaunchKernelGGL(gather_views_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, P, views);
+    hipLaunchKernelGGL(gather_rays_kernel, dim3(stride_blocks(pixels)), dim3(kBlock), 0, s, P, cotangent, rays);
+    if (out_fp64)
+        launch_voxels<double>(P, rays, views, guard, static_cast<double *>(out), accumulate ? 1 : 0, s);
+    else
+        launch_voxels<float>(P, rays, views, guard, static_cast<float *>(out), accumulate ? 1 : 0, s);
+    return hipGetLastError();
+}
+
+}  // namespace dpt

@@ -41,6 +41,7 @@
 #include "../../include/dpt_amd.h"
 
 #include "device_common.h"
+#include "gather.h"
 
 namespace dpt {
 
@@ -597,49 +598,6 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
 
 // --------------------------------------------------------- FP64 sampling --
 
-struct Interval64 {
-    double lower, upper;
-    int lo_axis, up_axis;
-    double lo_o, lo_d, up_o, up_d;
-};
-
-// kernels/projection.py:147-184
-__device__ __forceinline__ Interval64 interval64(const Params &P, const double *o, const double *d) {
-    Interval64 iv;
-    iv.lower = 0.0;
-    iv.upper = 1.0;
-    iv.lo_axis = iv.up_axis = -1;
-    iv.lo_o = iv.lo_d = iv.up_o = iv.up_d = 0.0;
-    const double highs[3] = {(double)P.nx - 0.5, (double)P.ny - 0.5, (double)P.nz - 0.5};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const double low = -0.5, high = highs[a], v = d[a];
-        if (v == 0.0) {
-            if (o[a] < low || o[a] > high) iv.upper = -1.0;
-        } else {
-            double first = (low - o[a]) / v, last = (high - o[a]) / v;
-            if (first > last) {
-                const double t = first;
-                first = last;
-                last = t;
-            }
-            if (first > iv.lower) {
-                iv.lower = first;
-                iv.lo_axis = a;
-                iv.lo_o = -1.0 / v;
-                iv.lo_d = -first / v;
-            }
-            if (last < iv.upper) {
-                iv.upper = last;
-                iv.up_axis = a;
-                iv.up_o = -1.0 / v;
-                iv.up_d = -last / v;
-            }
-        }
-    }
-    return iv;
-}
-
 struct Cell64 {
     int base;
     int lx, ly, lz;
@@ -762,28 +720,6 @@ __global__ __launch_bounds__(kBlock) void forward64_kernel(Params P, const T *__
     }
     out[ray] = integral;
     if (!isfinite(integral)) flag(P, view, DPT_STATUS_RANGE);
-}
-
-// kernels/numerics.py:11-56: product through a mantissa/exponent split
-__device__ __forceinline__ double product4(double a, double b, double c, double d) {
-    const double f[4] = {a, b, c, d};
-    double ordinary = a * b * c * d;
-    bool invalid = false, zero = false;
-    for (int i = 0; i < 4; ++i) {
-        invalid |= !isfinite(f[i]);
-        zero |= f[i] == 0.0;
-    }
-    if (invalid) return ordinary;
-    if (zero) return 0.0;
-    double m = 1.0;
-    int e = 0;
-    for (int i = 0; i < 4; ++i) {
-        int ei;
-        const double mi = frexp(f[i], &ei);
-        m = (i == 0) ? mi : m * mi;
-        e += ei;
-    }
-    return ldexp(m, e);
 }
 
 // kernels/projection.py:873-1237 (midpoint, FP64 sampling, FP32 increments)
@@ -1017,19 +953,7 @@ __global__ __launch_bounds__(kBlock) void check_finite_kernel(const T *__restric
 __global__ void check_poses_kernel(const double *__restrict__ poses, int views, int *status) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= views) return;
-    const double *R = poses + 12 * (size_t)v;
-    bool valid = isfinite(R[9]) && isfinite(R[10]) && isfinite(R[11]);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            if (!isfinite(R[3 * i + j])) valid = false;
-            // (R^T R)[i][j] = sum_k R[k][i] R[k][j]
-            const double prod = R[i] * R[j] + R[3 + i] * R[3 + j] + R[6 + i] * R[6 + j];
-            const double expected = (i == j) ? 1.0 : 0.0;
-            if (!(fabs(prod - expected) <= 1e-10)) valid = false;
-        }
-    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
-                       R[2] * (R[3] * R[7] - R[4] * R[6]);
-    if (det < 0.0) valid = false;
+    const bool valid = pose_is_rigid(poses + 12 * (size_t)v);
     if (!valid) atomicOr(&status[v], DPT_STATUS_POSE);
 }
 
@@ -1153,8 +1077,11 @@ struct dpt_plan {
     int vjp_vb = 1;          // views per workgroup of the VJP launches
     int forward_vb = 1;      // views per workgroup of the forward launch
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool timed[2] = {false, false};
+    GatherRay *d_gather_rays = nullptr;    // adjoint gather workspace, allocated at its first call
+    GatherView *d_gather_views = nullptr;
+    int *d_guard = nullptr;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool timed[3] = {false, false, false};
 };
 
 namespace {
@@ -1389,6 +1316,9 @@ int dpt_plan_destroy(dpt_plan *p) {
     (void)hipFree(p->d_alpha);
     (void)hipFree(p->d_cells);
     (void)hipFree(p->d_count);
+    (void)hipFree(p->d_gather_rays);
+    (void)hipFree(p->d_gather_views);
+    (void)hipFree(p->d_guard);
     if (p->h_status) (void)hipHostFree(p->h_status);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1648,6 +1578,38 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     return DPT_OK;
 }
 
+int dpt_adjoint_gather(dpt_plan *p, const double *poses, const double *cotangent, void *out, int32_t out_precision,
+                       uint32_t flags, void *stream) {
+    if (p == nullptr || poses == nullptr || cotangent == nullptr || out == nullptr)
+        return fail(DPT_E_INVALID, "plan, poses, cotangent and out are required");
+    if (p->arithmetic != DPT_ARITH_FP64) return fail(DPT_E_INVALID, "deterministic gather requires strict float64 sampling");
+    if (out_precision != DPT_ARITH_FP32 && out_precision != DPT_ARITH_FP64)
+        return fail(DPT_E_INVALID, "out_precision must be DPT_ARITH_FP32 or DPT_ARITH_FP64");
+    if (flags & ~(uint32_t)(DPT_ACCUMULATE | DPT_NO_CHECKS))
+        return fail(DPT_E_INVALID, "gather flags are DPT_ACCUMULATE and DPT_NO_CHECKS");
+    DeviceGuard guard(p->device);
+    if (p->d_gather_rays == nullptr) {
+        if (hipMalloc((void **)&p->d_gather_rays, sizeof(GatherRay) * (size_t)p->pixels) != hipSuccess)
+            return fail(DPT_E_MEMORY, "hipMalloc failed for the gather ray table");
+        if (hipMalloc((void **)&p->d_gather_views, sizeof(GatherView) * (size_t)p->base.views) != hipSuccess ||
+            hipMalloc((void **)&p->d_guard, sizeof(int)) != hipSuccess) {
+            (void)hipFree(p->d_gather_rays);
+            (void)hipFree(p->d_gather_views);
+            p->d_gather_rays = nullptr;
+            p->d_gather_views = nullptr;
+            return fail(DPT_E_MEMORY, "hipMalloc failed for the gather view table");
+        }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Params P = launch_params(p, poses);
+    timer_begin(p, 2, s);
+    HIP_TRY(launch_gather(P, cotangent, p->d_gather_rays, p->d_gather_views, p->d_guard, out,
+                          out_precision == DPT_ARITH_FP64, (flags & DPT_ACCUMULATE) != 0,
+                          (flags & DPT_NO_CHECKS) == 0, s));
+    timer_end(p, 2, s);
+    return DPT_OK;
+}
+
 int dpt_status(dpt_plan *p, int32_t *host_codes, int32_t clear, void *stream) {
     if (p == nullptr || host_codes == nullptr) return fail(DPT_E_INVALID, "plan and host_codes are required");
     DeviceGuard guard(p->device);
@@ -1694,12 +1656,12 @@ int dpt_convert_cells(const void *src, void *dst, int32_t nx, int32_t ny, int32_
 int dpt_set_timing(dpt_plan *p, int32_t enabled) {
     if (p == nullptr) return fail(DPT_E_INVALID, "plan is required");
     p->timing = enabled != 0;
-    p->timed[0] = p->timed[1] = false;
+    p->timed[0] = p->timed[1] = p->timed[2] = false;
     return DPT_OK;
 }
 
 double dpt_kernel_ms(dpt_plan *p, int32_t kind) {
-    if (p == nullptr || kind < 0 || kind > 1 || !p->timed[kind]) return -1.0;
+    if (p == nullptr || kind < 0 || kind > 2 || !p->timed[kind]) return -1.0;
     DeviceGuard guard(p->device);
     if (hipEventSynchronize(p->ev[2 * kind + 1]) != hipSuccess) return -1.0;
     float ms = -1.0f;

@@ -131,6 +131,25 @@ int dpt_vjp(dpt_plan *plan, const void *field, const double *poses, const void *
             float *grad_field, double *grad_pose, uint32_t flags, void *stream);
 
 /*
+ * Deterministic adjoint gather: out[v] = sum over views, pixels (row-major),
+ * midpoint samples and cell corners, in that order, of A(poses)[p, v] *
+ * cotangent[p], summed in FP64 by one thread per voxel, so the result is the
+ * same bit for bit on every call (unlike the atomic volume VJP).  Requires a
+ * DPT_ARITH_FP64 plan.  cotangent: float64 (views, rows, cols); out: (nz, ny,
+ * nx) float64 (out_precision DPT_ARITH_FP64) or float32 (DPT_ARITH_FP32),
+ * rounded once from the FP64 sum.  flags: DPT_ACCUMULATE adds the previous
+ * out[v] after the sum; DPT_NO_CHECKS skips the cotangent/pose validation.
+ * A non-finite cotangent (DPT_STATUS_RANGE) or a non-rigid pose
+ * (DPT_STATUS_POSE) leaves out untouched.  The plan allocates a ray table of
+ * 112 bytes per pixel at the first call.  Enqueued on `stream`, no host sync.
+ * Replaces prepare_projection_adjoint + projection_adjoint_gather
+ * (python/dpt/projection.py:1087-1405) and the kernels of
+ * kernels/projection_adjoint.py:73-170 and :296-635.
+ */
+int dpt_adjoint_gather(dpt_plan *plan, const double *poses, const double *cotangent, void *out,
+                       int32_t out_precision, uint32_t flags, void *stream);
+
+/*
  * Copy the per-view status words to host_codes (views int32) with one D2H
  * transfer and one stream synchronisation; clear them when `clear` != 0.
  * Replaces StatusCheckpoint.check/consume (python/dpt/_runtime.py:317-445).
@@ -157,7 +176,8 @@ int64_t dpt_cell_count(int32_t nx, int32_t ny, int32_t nz);
 int dpt_convert_cells(const void *src, void *dst, int32_t nx, int32_t ny, int32_t nz, int32_t storage,
                       void *stream);
 
-/* Kernel time of the most recent launch of `kind` (0 forward, 1 VJP) on the
+/* Kernel time of the most recent launch of `kind` (0 forward, 1 VJP,
+ * 2 adjoint gather including its ray preparation) on the
  * plan, in milliseconds, measured with HIP events on the launch stream;
  * negative when no timed launch has completed.  Enable with dpt_set_timing. */
 int dpt_set_timing(dpt_plan *plan, int32_t enabled);

@@ -731,3 +731,72 @@ void ref_set_threads(int n) {
     (void)n;
 #endif
 }
+
+/*
+ * Deterministic adjoint gather, kernels/projection_adjoint.py:73-170
+ * (get_gather, midpoint): out[v] = sum over views, pixels (row-major),
+ * samples and corners 0..7, in that order, of
+ * scaled_corner_weight(fraction, corner, volume_factor) (kernels/projection.py:
+ * 396-404: ((vf * w_x) * w_y) * w_z with no FP32 rounding) for every corner
+ * whose clamped index is v; FP64 throughout.
+ *
+ * The reference computes each voxel's sum by visiting every ray in that
+ * order.  A serial scatter over the same (view, pixel, sample, corner) order
+ * hands every voxel exactly the same sequence of addends, so it produces the
+ * same FP64 sums bit for bit at a cost of rays x samples instead of
+ * voxels x rays x samples; tests pin it against a voxel-driven Python
+ * restatement.  Single-threaded on purpose: the order is the contract.
+ *
+ * Rays skipped as in the reference: invalid (status RANGE), zero cotangent,
+ * empty clipped interval.  A non-finite sample point, or a non-finite final
+ * value, sets RANGE (the latter on status[0]).  With accumulate the previous
+ * out[v] is added after the local sum (projection_adjoint.py:164-166).
+ */
+void ref_gather_f64(const ref_plan *p, const double *poses, const double *cotangent, double *out,
+                    int accumulate, int32_t *status) {
+    const int shape[3] = {p->nx, p->ny, p->nz};
+    const int pixels = p->rows * p->cols;
+    const long total_rays = (long)p->views * pixels;
+    const size_t voxels = (size_t)p->nx * p->ny * p->nz;
+    double *total = (double *)calloc(voxels, sizeof(double));
+    for (long ray = 0; ray < total_rays; ++ray) {
+        int view = (int)(ray / pixels), pixel = (int)(ray % pixels);
+        ref_ray r;
+        ray_setup(p, poses + 12 * (size_t)view, view, pixel, &r);
+        if (!valid_ray(&r)) {
+            status[view] |= BIT_RANGE;
+            continue;
+        }
+        const double seed = cotangent[ray];
+        if (seed == 0.0) continue;
+        interval64 iv = finite_interval64(r.origin, r.direction, shape);
+        const double span = iv.upper - iv.lower;
+        if (span <= 0.0) continue;
+        const double factor = seed * r.length;
+        const double step = span / (double)p->samples;
+        double volume_factor = factor * span / (double)p->samples;
+        if (!isfinite(factor)) volume_factor = product4(seed, r.length, span, 1.0 / (double)p->samples);
+        for (int k = 0; k < p->samples; ++k) {
+            const double pos = iv.lower + ((double)k + 0.5) * step;
+            const double pt[3] = {r.origin[0] + pos * r.direction[0], r.origin[1] + pos * r.direction[1],
+                                  r.origin[2] + pos * r.direction[2]};
+            if (!finite3(pt)) status[view] |= BIT_RANGE;
+            if (!inside64(pt, shape)) continue;
+            int lower[3];
+            double f[3];
+            cell64(pt, shape, lower, f);
+            for (int corner = 0; corner < 8; ++corner) {
+                double contribution = volume_factor;
+                for (int a = 0; a < 3; ++a) contribution *= ((corner >> a) & 1) ? f[a] : 1.0 - f[a];
+                total[corner_index(lower, shape, corner)] += contribution;
+            }
+        }
+    }
+    for (size_t v = 0; v < voxels; ++v) {
+        double value = total[v];
+        if (accumulate) value += out[v];
+        out[v] = value;
+        if (!isfinite(value)) status[0] |= BIT_RANGE;
+    }
+    free(total);
+}

@@ -80,6 +80,8 @@ def _lib():
         for name in ("ref_vjp_f32", "ref_vjp_f64"):
             getattr(lib, name).argtypes = [ctypes.POINTER(_Plan), p, p, p, p, p, p, ctypes.c_int, p]
             getattr(lib, name).restype = None
+        lib.ref_gather_f64.argtypes = [ctypes.POINTER(_Plan), p, p, p, ctypes.c_int, p]
+        lib.ref_gather_f64.restype = None
         lib.ref_threads.restype = ctypes.c_int
         lib.ref_set_threads.argtypes = [ctypes.c_int]
         lib.ref_set_threads.restype = None
@@ -238,6 +240,120 @@ class Scene:
         fn(ctypes.byref(self.plan), field.ctypes.data, poses.ctypes.data, seeds.ctypes.data,
            ptr(grad_field), ptr(grad_pose), ptr(rays), components, status.ctypes.data)
         return grad_field, grad_pose, rays, status
+
+    def gather(self, cotangent, poses=None, *, previous=None, precision: str = "float64"):
+        """Deterministic FP64 adjoint gather (kernels/projection_adjoint.py:73-170).
+
+        Returns (out (voxels,) in ``precision``, status (V,) int32).  With
+        ``previous`` the call accumulates: out = dtype(total + float64(previous)).
+        """
+        poses = self._poses(poses)
+        seeds = np.ascontiguousarray(cotangent, dtype=np.float64).reshape(-1)
+        if seeds.size != self.views * self.rows * self.cols:
+            raise ValueError("cotangent size does not match image")
+        out = np.zeros(self.grid.voxels, dtype=np.float64)
+        if previous is not None:
+            out[:] = np.asarray(previous).reshape(-1).astype(np.float64)
+        status = np.zeros(self.views, dtype=np.int32)
+        _lib().ref_gather_f64(ctypes.byref(self.plan), poses.ctypes.data, seeds.ctypes.data, out.ctypes.data,
+                              int(previous is not None), status.ctypes.data)
+        if precision == "float32":
+            out = out.astype(np.float32)
+            if not np.isfinite(out).all():
+                status[0] |= 2
+        return out, status
+
+    def rays_grid(self, poses=None):
+        """Per-ray grid-frame (origin, direction, length) in the reference's
+        FP64 operation order (kernels/projection.py:79-102), for the
+        voxel-driven restatement below."""
+        poses = self._poses(poses)
+        g = self.grid.object_to_grid()
+        out = []
+        for view in range(self.views):
+            pose = [float(x) for x in poses[view]]
+            for pixel in range(self.rows * self.cols):
+                if self.plan.bundle:
+                    src = [float(x) for x in self._sources[pixel]]
+                    end = [float(x) for x in self._endpoints[pixel]]
+                else:
+                    pn = [float(x) for x in self._panels[view]]
+                    row, col = divmod(pixel, self.cols)
+                    src = pn[0:3]
+                    end = [pn[3 + i] + float(col) * pn[6 + i] + float(row) * pn[9 + i] for i in range(3)]
+                wsr = [src[i] - pose[9 + i] for i in range(3)]
+                wd = [end[i] - src[i] for i in range(3)]
+                osrc = [pose[i] * wsr[0] + pose[3 + i] * wsr[1] + pose[6 + i] * wsr[2] for i in range(3)]
+                odir = [pose[i] * wd[0] + pose[3 + i] * wd[1] + pose[6 + i] * wd[2] for i in range(3)]
+                rel = [osrc[i] - self.grid.origin[i] for i in range(3)]
+                origin = [g[3 * i] * rel[0] + g[3 * i + 1] * rel[1] + g[3 * i + 2] * rel[2] for i in range(3)]
+                direction = [g[3 * i] * odir[0] + g[3 * i + 1] * odir[1] + g[3 * i + 2] * odir[2]
+                             for i in range(3)]
+                length = math.sqrt(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2])
+                out.append((origin, direction, length))
+        return out
+
+
+def gather_voxel_driven(scene: Scene, cotangent, poses=None) -> np.ndarray:
+    """Voxel-driven restatement of get_gather (kernels/projection_adjoint.py:
+    79-166, midpoint, finite inputs): every voxel visits every ray, sample and
+    corner in order and keeps the addends whose clamped corner index is the
+    voxel (point_contribution, :33-44).  Pure Python; tiny grids only."""
+    nx, ny, nz = scene.grid.xyz
+    shape = (nx, ny, nz)
+    seeds = [float(x) for x in np.asarray(cotangent, dtype=np.float64).reshape(-1)]
+    rays = scene.rays_grid(poses)
+    samples = scene.samples
+
+    def interval(o, d):
+        lower, upper = 0.0, 1.0
+        for a in range(3):
+            low, high = -0.5, float(shape[a]) - 0.5
+            if d[a] == 0.0:
+                if o[a] < low or o[a] > high:
+                    upper = -1.0
+            else:
+                first, last = (low - o[a]) / d[a], (high - o[a]) / d[a]
+                if first > last:
+                    first, last = last, first
+                lower = max(lower, first) if first > lower else lower
+                upper = last if last < upper else upper
+        return lower, upper
+
+    out = np.zeros(nx * ny * nz, dtype=np.float64)
+    for voxel in range(nx * ny * nz):
+        total = 0.0
+        for ray, (o, d, length) in enumerate(rays):
+            seed = seeds[ray]
+            if seed == 0.0:
+                continue
+            lower, upper = interval(o, d)
+            span = upper - lower
+            if span <= 0.0:
+                continue
+            step = span / float(samples)
+            volume_factor = seed * length * span / float(samples)
+            for k in range(samples):
+                pos = lower + (float(k) + 0.5) * step
+                p = [o[a] + pos * d[a] for a in range(3)]
+                if not all(-0.5 <= p[a] <= float(shape[a]) - 0.5 for a in range(3)):
+                    continue
+                low, frac = [], []
+                for a in range(3):
+                    c = min(max(p[a], 0.0), float(shape[a] - 1))
+                    lo = min(int(math.floor(c)), max(shape[a] - 2, 0))
+                    low.append(lo)
+                    frac.append(c - float(lo))
+                for corner in range(8):
+                    idx = [min(low[a] + ((corner >> a) & 1), shape[a] - 1) for a in range(3)]
+                    if (idx[2] * ny + idx[1]) * nx + idx[0] != voxel:
+                        continue
+                    contribution = volume_factor
+                    for a in range(3):
+                        contribution *= frac[a] if (corner >> a) & 1 else 1.0 - frac[a]
+                    total += contribution
+        out[voxel] = total
+    return out
 
 
 # --------------------------------------------------------------------------

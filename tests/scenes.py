@@ -112,3 +112,32 @@ def smooth_field(grid: O.Grid, seed: int = 0) -> np.ndarray:
     z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     base = 0.02 + 0.01 * np.sin(0.3 * x + 0.1) * np.cos(0.2 * y) + 0.005 * np.cos(0.25 * z)
     return (base + 0.002 * rng.random(base.shape)).astype(np.float32)
+
+
+GATHER_CASES = ("nonsquare", "bundle", "clamped")
+
+
+def gather_scene(case: str):
+    """tests/python/warp_cpu/test_projection_adjoint_gather.py:26-69: a 2x5
+    panel at a 0.1 rad yaw, a six-ray bundle with axis-parallel, oblique and
+    missing rays, and a clamped 1x2x3 grid.  Returns (grid, geometry kwargs,
+    pose)."""
+    if case == "clamped":
+        grid = O.Grid((1, 2, 3), (1.0, 0.7, 1.2), (-1.0, 0.0, 0.0))
+        sources = np.array([(-4.0, -0.35, 0.0), (-4.0, 0.9, 0.0), (0.1, 0.1, 0.0), (-4.0, 9.0, 0.0)])
+        endpoints = np.array([(4.0, -0.35, 0.0), (4.0, 0.9, 0.0), (0.7, 0.3, 0.0), (4.0, 9.0, 0.0)])
+        return grid, dict(rays=(sources, endpoints)), IDENTITY12
+    grid = O.Grid((2, 3, 4), (1.2, 0.8, 1.1), (-1.8, -0.8, -0.55))
+    pose = compose(IDENTITY12, (0.07, -0.03, 0.02, 0.01, -0.02, 0.005))
+    if case == "bundle":
+        sources = np.array(
+            [(-8.0, 0.1, 0.2), (8.0, 0.1, 0.2), (-8.0, 9.0, 9.0), (0.1, 0.2, 0.3), (-1.0, -5.0, 0.2), (-0.7, 0.4, -8.0)]
+        )
+        endpoints = np.array(
+            [(8.0, 0.8, 0.7), (-8.0, 0.8, 0.7), (8.0, 9.0, 9.0), (0.3, 0.6, 0.7), (-1.0, 5.0, 0.3), (0.2, 0.4, 8.0)]
+        )
+        return grid, dict(rays=(sources, endpoints)), pose
+    panel = O.Panel(
+        (-8.0, 0.17, 0.23), (8.0, -0.43, -0.37), (math.sin(0.1), math.cos(0.1), 0.0), (0.0, 0.0, 1.0), (0.19, 0.23), (2, 5)
+    )
+    return grid, dict(panels=[panel]), pose

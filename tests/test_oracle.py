@@ -241,3 +241,97 @@ def test_oracle_flags_invalid_rays_and_poses():
     bad_pose[9] = math.nan
     _, status = scene.forward(np.ones(8, dtype=np.float32), bad_pose)
     assert status.tolist() == [2]
+
+
+# ---------------------------------------------------------------------------
+# Deterministic adjoint gather (kernels/projection_adjoint.py:73-170)
+# ---------------------------------------------------------------------------
+
+
+def _gather_case(case):
+    if case == "workflow":
+        grid, panels, _ = scenes.workflow_scene(3)
+        return grid, dict(panels=panels), scenes.compose(scenes.IDENTITY12, (0.03, -0.02, 0.01, 0.04, -0.03, 0.02))
+    return scenes.gather_scene(case)
+
+
+GATHER_ORACLE_CASES = [*scenes.GATHER_CASES, "workflow"]
+
+
+@pytest.mark.parametrize("case", GATHER_ORACLE_CASES)
+def test_gather_serial_order_equals_the_voxel_driven_restatement(case):
+    """The serial (view, pixel, sample, corner) scatter hands each voxel the
+    addends in the reference's per-voxel order, so the sums are identical."""
+    grid, geometry, pose = _gather_case(case)
+    scene = O.Scene(grid, 31, **geometry)
+    seeds = np.random.default_rng(584291).uniform(-1.0, 1.0, scene.views * scene.rows * scene.cols)
+    fast, status = scene.gather(seeds, pose)
+    assert status.tolist() == [0] * scene.views
+    slow = O.gather_voxel_driven(scene, seeds, pose)
+    np.testing.assert_array_equal(fast.view(np.int64), slow.view(np.int64))
+    assert np.count_nonzero(fast) > 0
+
+
+def _forward_matrix(scene, grid, pose):
+    columns = []
+    for voxel in range(grid.voxels):
+        basis = np.zeros(grid.voxels, dtype=np.float32)
+        basis[voxel] = 1.0
+        image, _ = scene.forward(basis, pose, precision="fp64")
+        columns.append(image.reshape(-1))
+    return np.stack(columns, axis=1)  # (pixels, voxels)
+
+
+@pytest.mark.parametrize("case", GATHER_ORACLE_CASES)
+def test_gather_is_the_transpose_of_the_fp64_forward_and_repeatable(case):
+    """tests/python/warp_cpu/test_projection_adjoint_gather.py:97-155: the
+    gather equals sum_p A[p,v] c[p] over the explicit FP64 forward matrix
+    (2e-12), satisfies <A mu, c> = <mu, A^T c>, repeats bit for bit and agrees
+    with the FP64 scatter VJP (whose increments are FP32-rounded)."""
+    grid, geometry, pose = _gather_case(case)
+    scene = O.Scene(grid, 31, **geometry)
+    matrix = _forward_matrix(scene, grid, pose)
+    rng = np.random.default_rng(584291)
+    for _ in range(3):
+        values = rng.uniform(-1.0, 1.0, grid.voxels).astype(np.float32)
+        seeds = rng.uniform(-1.0, 1.0, matrix.shape[0])
+        actual, _ = scene.gather(seeds, pose)
+        expected = np.array([math.fsum(matrix[:, v] * seeds) for v in range(grid.voxels)])
+        np.testing.assert_allclose(actual, expected, rtol=2e-12, atol=2e-12)
+        image, _ = scene.forward(values, pose, precision="fp64")
+        left = math.fsum(image.reshape(-1) * seeds)
+        right = math.fsum(values.astype(np.float64) * actual)
+        assert abs(left - right) < 2e-12 * max(1.0, abs(left), abs(right))
+        again, _ = scene.gather(seeds, pose)
+        assert again.tobytes() == actual.tobytes()
+        scatter, _, _, _ = scene.vjp(values, seeds, pose, precision="fp64", pose=False)
+        np.testing.assert_allclose(scatter, actual, rtol=1e-6, atol=1e-7 * np.abs(actual).max())
+
+
+def test_gather_accumulates_and_rounds_once_to_float32():
+    """tests/python/warp_cpu/test_projection_adjoint_gather.py:158-195."""
+    grid, geometry, pose = scenes.gather_scene("clamped")
+    scene = O.Scene(grid, 31, **geometry)
+    ones = np.ones(scene.views * scene.rows * scene.cols)
+    first, _ = scene.gather(ones, pose)
+    second, _ = scene.gather(ones, pose, previous=first)
+    assert second.tobytes() == (2.0 * first).tobytes()
+    single, _ = scene.gather(ones, pose, precision="float32")
+    assert single.tobytes() == first.astype(np.float32).tobytes()
+    again, _ = scene.gather(ones, pose, previous=single, precision="float32")
+    assert again.tobytes() == (first + single.astype(np.float64)).astype(np.float32).tobytes()
+
+
+def test_gather_flags_overflow_and_invalid_rays():
+    """tests/python/warp_cpu/test_projection_adjoint_gather.py:219-235 (1e308
+    cotangent accumulated onto 1e308) and an invalid pose translation."""
+    grid = O.Grid((1, 1, 1), (1.0, 1.0, 1.0))
+    scene = O.Scene(grid, 3, rays=(np.array([[0.0, 0.0, -2.0]]), np.array([[0.0, 0.0, 2.0]])))
+    out, status = scene.gather([1e308], previous=np.array([1e308]))
+    assert status.tolist() == [2] and not np.isfinite(out).all()
+    plain, status = scene.gather([1.0])
+    assert status.tolist() == [0] and plain[0] > 0.0
+    bad = list(scenes.IDENTITY12)
+    bad[9] = math.nan
+    _, status = scene.gather([1.0], bad)
+    assert status.tolist() == [2]
