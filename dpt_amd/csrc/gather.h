@@ -15,6 +15,7 @@ struct alignas(16) GatherRay {
     double o[3], d[3], inv[3];  // grid-frame origin, direction, 1/direction (0 where direction is 0)
     double lower, step, inv_step, vf, pad;
 };
+static_assert(sizeof(GatherRay) == 112, "a ray record is seven 16-byte loads");
 
 // Per-view projection of grid points onto the panel's (column, row)
 // coordinates for the detector culling: a point x lies on the ray of pixel

@@ -192,16 +192,38 @@ struct Voxel {
 // caller then visits every pixel).
 __device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &gv, const Voxel &vx, int &c0,
                                              int &c1, int &r0, int &r1) {
+    // Corner q of the widened box is w_lo + (q&1) e_x + (q>>1&1) e_y +
+    // (q>>2&1) e_z relative to the source; its three dot products are sums
+    // of per-axis terms (rounding differences are far inside the margins).
+    double den0 = 0.0, cn0 = 0.0, rn0 = 0.0, reach = 0.0;
+    double da[3], db[3], dc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double lo = vx.lo[a] - gv.margin - gv.s[a];
+        const double e = (vx.hi[a] - vx.lo[a]) + 2.0 * gv.margin;
+        den0 += lo * gv.a[a];
+        cn0 += lo * gv.b[a];
+        rn0 += lo * gv.c[a];
+        da[a] = e * gv.a[a];
+        db[a] = e * gv.b[a];
+        dc[a] = e * gv.c[a];
+        reach += fmax(fabs(lo), fabs(lo + e));  // |w|_1 bound over the box (>= |w|_2)
+    }
+    const double threshold = kFacing * gv.norm_a * reach;
     double cmin = INFINITY, cmax = -INFINITY, rmin = INFINITY, rmax = -INFINITY;
     bool good = true;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        double w[3];
-        for (int a = 0; a < 3; ++a) w[a] = (((q >> a) & 1) ? vx.hi[a] + gv.margin : vx.lo[a] - gv.margin) - gv.s[a];
-        const double den = dot3(w, gv.a);
-        good = good && den > kFacing * sqrt(dot3(w, w)) * gv.norm_a;
-        const double inv = 1.0 / den;
-        const double c = dot3(w, gv.b) * inv, r = dot3(w, gv.c) * inv;
+        const double den = den0 + ((q & 1) ? da[0] : 0.0) + ((q & 2) ? da[1] : 0.0) + ((q & 4) ? da[2] : 0.0);
+        const double cn = cn0 + ((q & 1) ? db[0] : 0.0) + ((q & 2) ? db[1] : 0.0) + ((q & 4) ? db[2] : 0.0);
+        const double rn = rn0 + ((q & 1) ? dc[0] : 0.0) + ((q & 2) ? dc[1] : 0.0) + ((q & 4) ? dc[2] : 0.0);
+        good = good && den > threshold;
+        // hardware reciprocal refined by two Newton steps (relative error far
+        // below the pixel margin; only used when den is positive and normal)
+        double inv = __builtin_amdgcn_rcp(den);
+        inv = fma(inv, fma(-den, inv, 1.0), inv);
+        inv = fma(inv, fma(-den, inv, 1.0), inv);
+        const double c = cn * inv, r = rn * inv;
         cmin = fmin(cmin, c);
         cmax = fmax(cmax, c);
         rmin = fmin(rmin, r);
@@ -226,51 +248,78 @@ __device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &
 // support box; false when none can.
 __device__ __forceinline__ bool sample_window(const Params &P, const GatherRay &g, const Voxel &vx, int &k0,
                                               int &k1) {
-    k0 = 0;
-    k1 = P.samples - 1;
+    // Branch-free slab clip of the widened box: lanes of a wave test
+    // different rays, so every lane runs the same instructions.
     double tlo = -INFINITY, thi = INFINITY;
-    bool full = false;
+    bool outside = false, full = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const double m = kBoxMargin * (1.0 + fabs(g.o[a]) + fabs(g.d[a]));
-        const double lo = vx.lo[a] - m, hi = vx.hi[a] + m;
-        if (g.d[a] == 0.0) {
-            if (g.o[a] < lo || g.o[a] > hi) return false;
-        } else {
-            double t1 = (lo - g.o[a]) * g.inv[a], t2 = (hi - g.o[a]) * g.inv[a];
-            if (isnan(t1) || isnan(t2)) {
-                full = true;
-            } else {
-                if (t1 > t2) {
-                    const double t = t1;
-                    t1 = t2;
-                    t2 = t;
-                }
-                tlo = fmax(tlo, t1);
-                thi = fmin(thi, t2);
-            }
-        }
+        const double lo = (vx.lo[a] - m) - g.o[a], hi = (vx.hi[a] + m) - g.o[a];
+        const bool flat = g.d[a] == 0.0;  // the coordinate is o on the whole ray
+        outside = outside || (flat && (lo > 0.0 || hi < 0.0));
+        const double t1 = lo * g.inv[a], t2 = hi * g.inv[a];
+        full = full || (!flat && (isnan(t1) || isnan(t2)));
+        tlo = flat ? tlo : fmax(tlo, fmin(t1, t2));
+        thi = flat ? thi : fmin(thi, fmax(t1, t2));
     }
-    if (full) return true;
-    if (tlo > thi) return false;
     tlo -= kParamMargin * (1.0 + fabs(tlo));
     thi += kParamMargin * (1.0 + fabs(thi));
-    if (!(g.step > kMinStep)) return true;
-    double kf0 = (tlo - g.lower) * g.inv_step - 0.5, kf1 = (thi - g.lower) * g.inv_step - 0.5;
-    if (isnan(kf0) || isnan(kf1)) return true;
-    kf0 = fmin(fmax(kf0, -2.0), (double)P.samples + 1.0);
-    kf1 = fmax(fmin(kf1, (double)P.samples + 1.0), -2.0);
-    if (kf0 > kf1) return false;
-    k0 = max(0, (int)floor(kf0) - 1);
-    k1 = min(P.samples - 1, (int)ceil(kf1) + 1);
-    return k0 <= k1;
+    // sample k lies in [tlo, thi] when kf0 <= k <= kf1; the box and t
+    // margins already exceed the point rounding by orders of magnitude, the
+    // slack only absorbs the rounding of kf itself
+    const double slack = 1e-6 + 1e-14 * g.inv_step;
+    double kf0 = (tlo - g.lower) * g.inv_step - 0.5 - slack, kf1 = (thi - g.lower) * g.inv_step - 0.5 + slack;
+    const bool exact = !full && g.step > kMinStep && !isnan(kf0) && !isnan(kf1);
+    const double top = (double)P.samples + 1.0;
+    kf0 = fmin(fmax(kf0, -2.0), top);
+    kf1 = fmin(fmax(kf1, -2.0), top);
+    const int a0 = max(0, (int)ceil(kf0)), a1 = min(P.samples - 1, (int)floor(kf1));
+    k0 = exact ? a0 : 0;
+    k1 = exact ? a1 : P.samples - 1;
+    return !outside && (!exact || (tlo <= thi && a0 <= a1));
 }
+
+// One ray record in seven 16-byte loads, issued together.
+__device__ __forceinline__ GatherRay load_ray(const GatherRay *r) {
+    const double2 *q = reinterpret_cast<const double2 *>(r);
+    const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+    GatherRay g;
+    g.o[0] = q0.x;
+    g.o[1] = q0.y;
+    g.o[2] = q1.x;
+    g.d[0] = q1.y;
+    g.d[1] = q2.x;
+    g.d[2] = q2.y;
+    g.inv[0] = q3.x;
+    g.inv[1] = q3.y;
+    g.inv[2] = q4.x;
+    g.lower = q4.y;
+    g.step = q5.x;
+    g.inv_step = q5.y;
+    g.vf = q6.x;
+    g.pad = q6.y;
+    return g;
+}
+
+// How a voxel's corner test is evaluated (all three give the reference's
+// addends bit for bit):
+//   kGeneral  any grid: the 8 corners of the sample's clamped cell in order,
+//             each compared with the voxel (clamped duplicates on singleton
+//             axes are added once per corner, as in the reference);
+//   kSimple   every axis has >= 2 samples: at most one corner is the voxel;
+//   kInterior every coordinate of the voxel is in [2, n-3]: a matching
+//             cell lies in [1, n-2) on every axis, where the support test
+//             and the clamps are the identity, and a point outside the
+//             support or clamped gives a lower corner of -1 or less, 0 or
+//             n-2 on some axis, none of which can match; so floor(p)
+//             replaces the clamped cell and the support test is dropped.
+constexpr int kGeneral = 0, kSimple = 1, kInterior = 2;
 
 // Add the contributions of ray g's samples k0..k1 to the voxel's total, in
 // sample then corner order (kernels/projection_adjoint.py:158-163 with
-// point_contribution, :33-44).  SIMPLE: every axis has >= 2 samples, so at
-// most one corner of a cell is the voxel.
-template <bool SIMPLE>
+// point_contribution, :33-44).
+template <int MODE>
 __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, const Voxel &vx, int k0, int k1,
                                           double total) {
     const int n[3] = {P.nx, P.ny, P.nz};
@@ -280,6 +329,19 @@ __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, c
         const double pos = g.lower + ((double)k + 0.5) * g.step;
         double p[3];
         for (int a = 0; a < 3; ++a) p[a] = g.o[a] + pos * g.d[a];
+        if constexpr (MODE == kInterior) {
+            bool hit = true;
+            double w[3];
+            for (int a = 0; a < 3; ++a) {
+                const double lower = floor(p[a]);
+                const int l = (int)lower;
+                const double f = p[a] - lower;
+                hit = hit && (l == vx.i[a] || l + 1 == vx.i[a]);
+                w[a] = l == vx.i[a] ? 1.0 - f : f;
+            }
+            if (hit) total += ((vf * w[0]) * w[1]) * w[2];
+            continue;
+        }
         // kernels/projection.py:124-134 (inside_support; NaN fails every test)
         if (!(p[0] >= -0.5 && p[1] >= -0.5 && p[2] >= -0.5 && p[0] <= (double)n[0] - 0.5 &&
               p[1] <= (double)n[1] - 0.5 && p[2] <= (double)n[2] - 0.5))
@@ -292,7 +354,7 @@ __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, c
             l[a] = min((int)floor(c), cap[a]);
             f[a] = c - (double)l[a];
         }
-        if constexpr (SIMPLE) {
+        if constexpr (MODE == kSimple) {
             bool hit = true;
             double w[3];
             for (int a = 0; a < 3; ++a) {
@@ -317,33 +379,10 @@ __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, c
     return total;
 }
 
-// Voxel bricks of 8 x 8 x 4 per workgroup (4 x 4 x 4 per wave, so a wave's
-// voxels see nearly the same rays); consecutive workgroups go to different
-// XCDs, so each XCD takes a contiguous range of bricks and its L2 keeps the
-// ray records of neighbouring bricks.
-template <typename OUT, bool SIMPLE>
-__global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRay *__restrict__ rays,
-                                                        const GatherView *__restrict__ views,
-                                                        const int *__restrict__ guard, OUT *__restrict__ out,
-                                                        int accumulate, int bricks_x, int bricks_y, int bricks,
-                                                        int chunk) {
-    if (*guard) return;  // a failed check: the destination stays untouched
-    const int b = blockIdx.x;
-    const int logical = (b % kXcds) * chunk + b / kXcds;
-    if (logical >= bricks) return;
-    const int bx = logical % bricks_x, rest = logical / bricks_x;
-    const int by = rest % bricks_y, bz = rest / bricks_y;
-    const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    Voxel vx;
-    vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
-    vx.i[1] = by * 8 + (w >> 1) * 4 + ((l >> 2) & 3);
-    vx.i[2] = bz * 4 + (l >> 4);
-    const int n[3] = {P.nx, P.ny, P.nz};
-    if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;  // no barriers below
-    for (int a = 0; a < 3; ++a) {
-        vx.lo[a] = vx.i[a] <= 1 ? -0.5 : (double)(vx.i[a] - 1);
-        vx.hi[a] = vx.i[a] >= n[a] - 2 ? (double)n[a] - 0.5 : (double)(vx.i[a] + 1);
-    }
+// The voxel's FP64 sum over every view, pixel window and sample window.
+template <int MODE>
+__device__ __forceinline__ double gather_voxel(const Params &P, const GatherRay *__restrict__ rays,
+                                               const GatherView *__restrict__ views, const Voxel &vx) {
     const int64_t pixels = (int64_t)P.rows * P.cols;
     double total = 0.0;
     for (int view = 0; view < P.views; ++view) {
@@ -360,12 +399,53 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRa
         const GatherRay *base = rays + (int64_t)view * pixels;
         for (int row = r0; row <= r1; ++row)
             for (int col = c0; col <= c1; ++col) {
-                const GatherRay &g = base[(int64_t)row * P.cols + col];
-                if (g.vf == 0.0) continue;
+                const GatherRay g = load_ray(base + (int64_t)row * P.cols + col);
                 int k0, k1;
-                if (!sample_window(P, g, vx, k0, k1)) continue;
-                total = add_ray<SIMPLE>(P, g, vx, k0, k1, total);
+                const bool reach = sample_window(P, g, vx, k0, k1);
+                if (g.vf != 0.0 && reach) total = add_ray<MODE>(P, g, vx, k0, k1, total);
             }
+    }
+    return total;
+}
+
+// Voxel bricks of 8 x 8 x 4 per workgroup (4 x 4 x 4 per wave, so a wave's
+// voxels see nearly the same rays); consecutive workgroups go to different
+// XCDs, so each XCD takes a contiguous range of bricks and its L2 keeps the
+// ray records of neighbouring bricks.
+template <typename OUT, bool SIMPLE>
+__global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRay *__restrict__ rays,
+                                                        const GatherView *__restrict__ views,
+                                                        const int *__restrict__ guard, OUT *__restrict__ out,
+                                                        int accumulate, int bricks_x, int bricks_y, int bricks,
+                                                        int chunk) {
+    if (*guard) return;  // a failed check: the destination stays untouched (uniform)
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * chunk + b / kXcds;
+    if (logical >= bricks) return;  // uniform per workgroup
+    const int bx = logical % bricks_x, rest = logical / bricks_x;
+    const int by = rest % bricks_y, bz = rest / bricks_y;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    Voxel vx;
+    vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
+    vx.i[1] = by * 8 + (w >> 1) * 4 + ((l >> 2) & 3);
+    vx.i[2] = bz * 4 + (l >> 4);
+    const int n[3] = {P.nx, P.ny, P.nz};
+    if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;  // no barriers below
+    for (int a = 0; a < 3; ++a) {
+        vx.lo[a] = vx.i[a] <= 1 ? -0.5 : (double)(vx.i[a] - 1);
+        vx.hi[a] = vx.i[a] >= n[a] - 2 ? (double)n[a] - 0.5 : (double)(vx.i[a] + 1);
+    }
+    double total;
+    if constexpr (SIMPLE) {
+        bool interior = true;
+        for (int a = 0; a < 3; ++a) interior = interior && vx.i[a] >= 2 && vx.i[a] <= n[a] - 3;
+        // one path per wave: bricks on the grid boundary take the checked one
+        if (__all(interior))
+            total = gather_voxel<kInterior>(P, rays, views, vx);
+        else
+            total = gather_voxel<kSimple>(P, rays, views, vx);
+    } else {
+        total = gather_voxel<kGeneral>(P, rays, views, vx);
     }
     const int64_t voxel = ((int64_t)vx.i[2] * n[1] + vx.i[1]) * n[0] + vx.i[0];
     if (accumulate) total += (double)out[voxel];

@@ -8,6 +8,9 @@ Same operator contract as the reference (python/dpt/torch/_projection_ops.py:
   active_pose) -> (grad_field | empty(0), grad_poses | empty(0))``
 * ``dpt_amd::project_out(field, poses, value, out!)``
 
+plus ``adjoint_gather``, the deterministic FP64 transpose (an explicit,
+non-autograd call).
+
 ``value`` is the JSON configuration.  Fakes make the ops traceable; autograd
 is first order only.  Work is enqueued on the caller's current stream (one
 native plan per stream), followed by exactly one status readback per call.
@@ -167,6 +170,45 @@ def _backward(ctx, cotangent):
 
 
 project.register_autograd(_backward, setup_context=_setup_context)
+
+
+def adjoint_gather(poses: torch.Tensor, cotangent: torch.Tensor, value: str, out: torch.Tensor,
+                   accumulate: bool) -> torch.Tensor:
+    """Deterministic FP64 transpose: out[v] (+)= sum_p A(poses)[p, v] cotangent[p].
+
+    Same contract as projection_adjoint_gather (python/dpt/projection.py:
+    1253-1405) for a multiview plan: strict float64 sampling, a float64
+    cotangent of the image shape, out float64 or float32 with the grid's voxel
+    count, accumulate adding the previous out after each voxel's FP64 sum.
+    Invalid input raises before anything is written.
+    """
+    decoded = decode(value)
+    if decoded.spec.precision != "float64":
+        raise ContractError("deterministic gather requires strict float64 sampling")
+    if type(accumulate) is not bool:
+        raise ContractError("accumulate must be a boolean")
+    device = cotangent.device
+    if tuple(cotangent.shape) != decoded.image_shape or cotangent.dtype != torch.float64:
+        raise ContractError("cotangent must be float64 with the projected image shape")
+    if tuple(poses.shape) != (decoded.views, 12) or poses.dtype != torch.float64 or poses.device != device:
+        raise ContractError("poses must be float64 (views,12) on the cotangent device")
+    if out.dtype not in (torch.float32, torch.float64) or out.numel() != decoded.grid.voxels:
+        raise ContractError("out must be float32 or float64 with one element per voxel")
+    if out.device != device or not out.is_contiguous():
+        raise ContractError("out must be contiguous on the cotangent device")
+    if any(out.untyped_storage().data_ptr() == x.untyped_storage().data_ptr() for x in (poses, cotangent)):
+        raise ContractError("out must not share storage with the gather inputs")
+    _require_device(cotangent)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    entry = prepare(value, device.index or 0, stream, 0)
+    with entry.lock, torch.no_grad():
+        packed = poses.contiguous()
+        seed = cotangent.contiguous()
+        precision = "float64" if out.dtype == torch.float64 else "float32"
+        flags = _native.ACCUMULATE if accumulate else 0
+        entry.plan.adjoint_gather(packed.data_ptr(), seed.data_ptr(), out.data_ptr(), precision, flags, stream)
+        consume(entry, stream)
+    return out
 
 
 @torch.library.custom_op("dpt_amd::project_out", mutates_args={"out"}, tags=_TAGS)

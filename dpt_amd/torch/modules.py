@@ -183,3 +183,37 @@ class Projector(torch.nn.Module):
             field.requires_grad and torch.is_grad_enabled(),
             poses.requires_grad and torch.is_grad_enabled(),
         )
+
+    def adjoint_gather(
+        self,
+        cotangent: torch.Tensor,
+        pose: torch.Tensor | None = None,
+        *,
+        packed_pose=None,
+        out: torch.Tensor | None = None,
+        volume_gradient_precision: str = "float64",
+        accumulate: bool = False,
+    ) -> torch.Tensor:
+        """Deterministic volume adjoint: out[v] = sum_p A(pose)[p, v] c[p],
+        each voxel summed in FP64 in a fixed (view, pixel, sample, corner)
+        order, so repeated calls agree bit for bit.  Requires
+        ``precision="fp64"``; ``cotangent`` is float64 (views, rows, columns);
+        the result has the grid shape (nz, ny, nx) in
+        ``volume_gradient_precision`` ("float64" or "float32").  With
+        ``accumulate`` the sums are added to ``out``.  Same contract as
+        projection_adjoint_gather (python/dpt/projection.py:1253-1405)."""
+        if self.precision != "fp64":
+            raise ContractError("deterministic gather requires strict float64 sampling (precision='fp64')")
+        if volume_gradient_precision not in ("float32", "float64"):
+            raise ContractError("volume_gradient_precision must be float32 or float64")
+        if not isinstance(cotangent, torch.Tensor):
+            raise ContractError("cotangent must be a Torch tensor")
+        poses = self._poses(cotangent, pose, packed_pose)
+        dtype = torch.float64 if volume_gradient_precision == "float64" else torch.float32
+        if out is None:
+            if accumulate:
+                raise ContractError("accumulate needs an out tensor")
+            out = torch.empty(self.grid.shape, dtype=dtype, device=cotangent.device)
+        elif out.dtype != dtype:
+            raise ContractError("out dtype must match volume_gradient_precision")
+        return _projection_ops.adjoint_gather(poses.contiguous(), cotangent, self._configuration, out, accumulate)

@@ -176,3 +176,42 @@ def test_gather_matches_adjoint_identity_and_scatter_at_size():
     assert status.tolist() == [0] * 24
     scale = np.abs(gathered).max()
     np.testing.assert_allclose(scatter, gathered, rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_projector_adjoint_gather_matches_the_oracle_and_the_module_adjoint():
+    """Projector.adjoint_gather: FP64 and FP32 destinations, accumulation,
+    and the contract errors of projection_adjoint_gather."""
+    from dpt_amd import phantoms
+    from dpt_amd.torch import Projector
+
+    grid = phantoms.centred_grid((14, 15, 16))
+    geometry, _ = phantoms.circular_geometry(grid, 3, (12, 13), sod=60.0, sdd=100.0)
+    module = Projector(grid=grid, geometry=geometry, samples_per_ray=40, precision="fp64")
+    xi = torch.tensor([0.2, -0.1, 0.3, 0.01, 0.02, -0.015], dtype=torch.float64, device="cuda")
+    gen = torch.Generator("cuda").manual_seed(4)
+    c = torch.randn(module.image_shape, dtype=torch.float64, device="cuda", generator=gen)
+    out = module.adjoint_gather(c, xi)
+    assert out.shape == grid.shape and out.dtype == torch.float64
+    poses = module._poses(c, xi, None)
+    ref = O.Scene(
+        O.Grid(grid.shape, grid.spacing_mm, grid.origin_mm, grid.orientation),
+        40,
+        panels=[O.Panel(p.source_mm, p.origin_mm, p.u, p.v, p.spacing_mm, p.shape) for p in geometry.views],
+    )
+    expected, _ = ref.gather(c.cpu().numpy(), poses.cpu().numpy())
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(-1).view(np.int64), expected.view(np.int64))
+    single = module.adjoint_gather(c, xi, volume_gradient_precision="float32")
+    np.testing.assert_array_equal(single.cpu().numpy().reshape(-1), expected.astype(np.float32))
+    module.adjoint_gather(c, xi, out=out, accumulate=True)
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(-1), 2.0 * expected)
+    # same transpose as the module's autograd adjoint (atomic, FP32-rounded increments)
+    field = torch.zeros(grid.shape, dtype=torch.float32, device="cuda", requires_grad=True)
+    (adjoint,) = torch.autograd.grad(module(field, xi), field, c)
+    scale = float(np.abs(expected).max())
+    np.testing.assert_allclose(adjoint.cpu().numpy().reshape(-1), expected, rtol=1e-5, atol=1e-6 * scale)
+    with pytest.raises(ContractError, match="float64"):
+        Projector(grid=grid, geometry=geometry, samples_per_ray=40).adjoint_gather(c.float(), xi)
+    with pytest.raises(ContractError, match="cotangent"):
+        module.adjoint_gather(c.float(), xi)
+    with pytest.raises(ContractError, match="accumulate"):
+        module.adjoint_gather(c, xi, accumulate=True)
