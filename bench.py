@@ -54,6 +54,9 @@ def _args():
     parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
     parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
     parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
+    parser.add_argument("--no-gather", action="store_true", help="skip the deterministic adjoint gather measurement")
+    parser.add_argument("--gather-only", action="store_true",
+                        help="only the deterministic adjoint gather measurement (profiling driver)")
     parser.add_argument("--transport-only", action="store_true",
                         help="only the Monte Carlo transport measurement (profiling driver)")
     parser.add_argument("--transport-histories", type=int, default=100_000_000)
@@ -362,6 +365,61 @@ def precision_benchmark(grid, geometry, values, target, samples_list=(512, 1024)
     return out
 
 
+def gather_benchmark(grid, geometry, values, target, samples: int) -> dict:
+    """Deterministic FP64 adjoint gather of the full scan: the cotangent is the
+    FP64 residual against the seeded target; the gather (one thread per
+    voxel, fixed-order FP64 sums) is timed on its second call (the first
+    allocates the plan's ray table), repeated once to check bit-identity,
+    and compared with the atomic FP64 volume VJP of the same cotangent."""
+    import torch
+
+    from dpt_amd.projection import prepare
+    from dpt_amd.torch import Projector
+
+    device = torch.device("cuda:0")
+    module = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)
+    pose = torch.zeros(6, dtype=torch.float64, device=device)
+    field = torch.tensor(values, device=device)
+    with torch.no_grad():
+        cotangent = (module(field, pose) - target.double()).contiguous()
+    entry = prepare(module._configuration, 0, torch.cuda.current_stream(device).cuda_stream, 0)
+    entry.plan.set_timing(True)
+    times, kernel = [], []
+    outputs = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outputs.append(module.adjoint_gather(cotangent, pose))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        kernel.append(entry.plan.kernel_ms(2))
+    repeat = bool(torch.equal(outputs[0], outputs[1]))
+    leaf = field.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    (atomic,) = torch.autograd.grad(module(leaf, pose), leaf, cotangent)
+    torch.cuda.synchronize()
+    atomic_s = time.perf_counter() - t0
+    a, b = outputs[1].reshape(-1), atomic.double().reshape(-1)
+    rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
+    voxels, views = grid.voxels, len(geometry.views)
+    result = {
+        "ms": round(times[1] * 1e3, 3),
+        "kernel_ms": round(kernel[1], 3),
+        "first_call_ms": round(times[0] * 1e3, 3),
+        "voxel_views_per_s": round(voxels * views / times[1], 1),
+        "bitwise_repeat": repeat,
+        "rel_l2_vs_atomic_fp64_volume_vjp": rel,
+        "atomic_fp64_forward_volume_vjp_ms": round(atomic_s * 1e3, 3),
+        "output": "float64 (nz, ny, nx)",
+        "note": "FP64 VALU-issue bound (one thread per voxel, culled pixel and sample windows); "
+                "ms includes the per-call ray preparation",
+    }
+    _log(f"gather: {result['ms']} ms, repeat bitwise {repeat}, rel vs atomic {rel:.2e}")
+    del outputs, atomic
+    return result
+
+
 def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -537,6 +595,9 @@ def main() -> int:
         target = make()(field.detach(), torch.tensor(xi_true, dtype=torch.float64, device=device))
         target += 1e-3 * torch.randn(target.shape, generator=torch.Generator(device).manual_seed(11), device=device)
     torch.cuda.synchronize()
+    if args.gather_only:
+        print(json.dumps(gather_benchmark(grid, geometry, values, target, args.samples)), flush=True)
+        return 0
 
     pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
     projector = make()
@@ -688,6 +749,9 @@ def main() -> int:
     if not args.no_precision:
         _log("precision benchmark")
         result["precision"] = precision_benchmark(grid, geometry, values, target)
+    if not args.no_gather:
+        _log("deterministic adjoint gather")
+        result["gather"] = gather_benchmark(grid, geometry, values, target, args.samples)
     if not args.no_transport:
         _log("transport benchmark")
         result["transport"] = transport_benchmark(args.transport_histories, args.transport_cpu_histories,
