@@ -52,6 +52,8 @@ STORE_CELLS = 8
 VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
 # plan option (needs): FP32 sampling gathers from a cell-layout copy
 GATHER_CELLS = 64
+# plan option (needs, FP64 only): exact cell-Gauss integration
+CELL_GAUSS = 128
 
 STORAGE_CODES = {"float32": STORE_F32, "bfloat16": STORE_BF16, "float16": STORE_F16}
 # 16-bit field buffers carry DPT_HALF_PAD_BYTES (8) of readable padding

@@ -260,6 +260,28 @@ __device__ __forceinline__ double product4(double a, double b, double c, double 
     return ldexp(m, e);
 }
 
+// kernels/numerics.py:11-56 for three factors
+__device__ __forceinline__ double product3(double a, double b, double c) {
+    const double f[3] = {a, b, c};
+    double ordinary = a * b * c;
+    bool invalid = false, zero = false;
+    for (int i = 0; i < 3; ++i) {
+        invalid |= !isfinite(f[i]);
+        zero |= f[i] == 0.0;
+    }
+    if (invalid) return ordinary;
+    if (zero) return 0.0;
+    double m = 1.0;
+    int e = 0;
+    for (int i = 0; i < 3; ++i) {
+        int ei;
+        const double mi = frexp(f[i], &ei);
+        m = (i == 0) ? mi : m * mi;
+        e += ei;
+    }
+    return ldexp(m, e);
+}
+
 
 // kernels/batched_geometry.py:71-88 + kernels/geometry.py:55-74: finite
 // entries, orthonormal to 1e-10 and a proper rotation.

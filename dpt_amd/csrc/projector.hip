@@ -897,6 +897,353 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
     }
 }
 
+// ------------------------------------------------ exact cell-Gauss (FP64) --
+// kernels/projection.py:522-746: the clipped ray is cut at every
+// interpolation plane it crosses and each piece gets two Gauss nodes, exact
+// for the cubic a trilinear field is along a line inside one cell.  Same
+// operation order as the oracle, so images agree bit for bit.
+
+struct CellWalk {
+    int plane[3], step[3];
+    double next[3], inv[3];
+};
+
+struct CellPiece {
+    double end, radius, first_position, second_position;
+    double first[3], second[3];
+    int lower[3];
+    bool shared;
+};
+
+// kernels/projection.py:629-640: cached reciprocal with one FMA residual
+// correction; range extremes keep the division
+__device__ __forceinline__ double crossing_quotient(double delta, double direction, double inverse) {
+    if (inverse == 0.0) return delta / direction;
+    const double q = delta * inverse;
+    if (!isfinite(q) || fabs(q) < 2.2250738585072014e-308) return delta / direction;
+    return fma(fma(-q, direction, delta), inverse, q);
+}
+
+// kernels/projection.py:643-655
+__device__ __forceinline__ double cell_crossing(const Ray &r, int a, int plane, int n, double upper, double inv) {
+    if (r.d[a] == 0.0 || plane < 0 || plane >= n) return upper;
+    const double c = crossing_quotient((double)plane - r.o[a], r.d[a], inv);
+    return c < upper ? c : upper;
+}
+
+// kernels/projection.py:659-695 (begin_cells)
+__device__ __forceinline__ void begin_cells(const Params &P, const Ray &r, const Interval64 &iv, CellWalk &w) {
+    const int n[3] = {P.nx, P.ny, P.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double entry = r.o[a] + iv.lower * r.d[a];
+        const double v = r.d[a];
+        w.inv[a] = 0.0;
+        w.step[a] = 0;
+        w.plane[a] = 0;
+        if (v != 0.0) {
+            const double inverse = 1.0 / v;
+            if (isfinite(inverse) && fabs(inverse) >= 2.2250738585072014e-308 && fabs(v) >= 2.2250738585072014e-308)
+                w.inv[a] = inverse;
+        }
+        if (v > 0.0) {
+            w.step[a] = 1;
+            w.plane[a] = max((int)ceil(entry), 0);
+        } else if (v < 0.0) {
+            w.step[a] = -1;
+            w.plane[a] = min((int)floor(entry), n[a] - 1);
+        }
+        double crossing = cell_crossing(r, a, w.plane[a], n[a], iv.upper, w.inv[a]);
+        if (crossing <= iv.lower) {
+            w.plane[a] += w.step[a];
+            crossing = cell_crossing(r, a, w.plane[a], n[a], iv.upper, w.inv[a]);
+        }
+        w.next[a] = crossing;
+    }
+}
+
+// kernels/projection.py:710-742 (cell_segment)
+__device__ __forceinline__ void cell_piece(const Params &P, const Ray &r, const CellWalk &w, double start,
+                                           CellPiece &s) {
+    const double m = w.next[1] < w.next[2] ? w.next[1] : w.next[2];
+    s.end = w.next[0] < m ? w.next[0] : m;
+    s.radius = (s.end - start) * 0.5;
+    const double centre = start + s.radius;
+    const double offset = s.radius / sqrt(3.0);
+    s.first_position = centre - offset;
+    s.second_position = centre + offset;
+    double mid[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.first[a] = r.o[a] + s.first_position * r.d[a];
+        s.second[a] = r.o[a] + s.second_position * r.d[a];
+        mid[a] = r.o[a] + centre * r.d[a];
+    }
+    lower_cell64(P, mid, s.lower);
+    const int n[3] = {P.nx, P.ny, P.nz};
+    s.shared = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double lo = (double)s.lower[a], up = lo + 1.0, top = (double)(n[a] - 1);
+        const double c1 = fmin(fmax(s.first[a], 0.0), top), c2 = fmin(fmax(s.second[a], 0.0), top);
+        if (c1 < lo || c2 < lo) s.shared = false;
+        if (s.lower[a] < n[a] - 2 && (c1 >= up || c2 >= up)) s.shared = false;
+    }
+}
+
+// kernels/projection.py:745-755 (advance_cells): every tied plane advances
+__device__ __forceinline__ void advance_cells(const Params &P, const Ray &r, CellWalk &w, double end, double upper) {
+    const int n[3] = {P.nx, P.ny, P.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (w.next[a] <= end) {
+            w.plane[a] += w.step[a];
+            w.next[a] = cell_crossing(r, a, w.plane[a], n[a], upper, w.inv[a]);
+        }
+}
+
+__device__ __forceinline__ bool inside_support64(const Params &P, const double *p) {
+    return p[0] >= -0.5 && p[1] >= -0.5 && p[2] >= -0.5 && p[0] <= (double)P.nx - 0.5 &&
+           p[1] <= (double)P.ny - 0.5 && p[2] <= (double)P.nz - 0.5;
+}
+
+// Value (and slope) of one node: sample_cell in the piece's cell when it is
+// shared (kernels/projection.py:278-284), else sample_field (:288-295); zero
+// outside the support.
+template <typename T, bool GRAD>
+__device__ __forceinline__ double node_value(const Params &P, const T *field, const double *pt, const int *lower,
+                                             double *g) {
+    if (GRAD) g[0] = g[1] = g[2] = 0.0;
+    Cell64 c;
+    if (!cell64(P, pt, c)) return 0.0;
+    if (lower != nullptr) {
+        c.lx = lower[0];
+        c.ly = lower[1];
+        c.lz = lower[2];
+        c.fx = fmin(fmax(pt[0], 0.0), (double)(P.nx - 1)) - (double)c.lx;
+        c.fy = fmin(fmax(pt[1], 0.0), (double)(P.ny - 1)) - (double)c.ly;
+        c.fz = fmin(fmax(pt[2], 0.0), (double)(P.nz - 1)) - (double)c.lz;
+        c.base = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+    }
+    double v[8];
+    load8d<T>(P, field, c.base, v);
+    return sample_cell64<GRAD>(c, v, g);
+}
+
+// Grid index of corner k of the cell at lower (kernels/projection.py:217-222)
+__device__ __forceinline__ int corner_of(const Params &P, const int *lower, int k) {
+    const int x = min(lower[0] + (k & 1), P.nx - 1);
+    const int y = min(lower[1] + ((k >> 1) & 1), P.ny - 1);
+    const int z = min(lower[2] + ((k >> 2) & 1), P.nz - 1);
+    return (z * P.ny + y) * P.nx + x;
+}
+
+// kernels/projection.py:802-846 with cell_gauss: integral * length
+template <typename T>
+__global__ __launch_bounds__(kBlock) void forward_gauss_kernel(Params P, const T *__restrict__ field,
+                                                               double *__restrict__ out) {
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
+    if (!ray_in_range(P, view, row, col)) return;
+    const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+    const double *pose = P.poses + 12 * (size_t)view;
+    World w;
+    ray_world(P, pose, view, row, col, w);
+    Ray r;
+    ray_grid(P, w, r);
+    if (!r.valid) {
+        flag(P, view, DPT_STATUS_RANGE);
+        out[ray] = 0.0;
+        return;
+    }
+    const Interval64 iv = interval64(P, r.o, r.d);
+    double integral = 0.0;
+    if (iv.upper > iv.lower) {
+        CellWalk walk;
+        begin_cells(P, r, iv, walk);
+        double start = iv.lower;
+        const long long limit = (long long)P.nx + P.ny + P.nz + 1;
+        bool bad = false;
+        for (long long segments = 0; start < iv.upper && segments < limit; ++segments) {
+            CellPiece s;
+            cell_piece(P, r, walk, start, s);
+            if (s.end <= start) {
+                bad = true;
+                break;
+            }
+            bad |= !finite3d(s.first) || !finite3d(s.second);
+            const int *lw = s.shared ? s.lower : nullptr;
+            integral += s.radius * node_value<T, false>(P, field, s.first, lw, nullptr);
+            integral += s.radius * node_value<T, false>(P, field, s.second, lw, nullptr);
+            start = s.end;
+            advance_cells(P, r, walk, s.end, iv.upper);
+        }
+        bad |= start < iv.upper;
+        if (bad) flag(P, view, DPT_STATUS_RANGE);
+        integral *= r.len;
+    }
+    out[ray] = integral;
+    if (!isfinite(integral)) flag(P, view, DPT_STATUS_RANGE);
+}
+
+// FP32-rounded increment of one corner into the FP64 accumulator (zeros add
+// nothing and are skipped)
+__device__ __forceinline__ bool add_increment(double *acc, int index, double contribution) {
+    const float rounded = (float)contribution;
+    if (rounded != 0.0f) unsafeAtomicAdd(acc + index, (double)rounded);
+    return isfinite(rounded);
+}
+
+// kernels/projection.py:873-1237 with cell_gauss (FP32 volume destination:
+// each increment rounded to FP32, summed here in FP64)
+template <typename T, bool VOL, bool POSE, int C>
+__global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__restrict__ field,
+                                                           const double *__restrict__ seeds,
+                                                           double *__restrict__ acc,
+                                                           double *__restrict__ partials) {
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
+    double gradient[kMaxComponents];
+#pragma unroll
+    for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
+    const double *pose = P.poses + 12 * (size_t)view;
+    if (ray_in_range(P, view, row, col)) {
+        const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        if (!r.valid) {
+            flag(P, view, DPT_STATUS_RANGE);
+        } else {
+            const Interval64 iv = interval64(P, r.o, r.d);
+            double span = iv.upper - iv.lower;
+            const double seed = seeds[ray];
+            if (seed == 0.0) span = 0.0;
+            if (span > 0.0) {
+                const double factor = seed * r.len;
+                double og[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
+                CellWalk walk;
+                begin_cells(P, r, iv, walk);
+                double start = iv.lower;
+                const long long limit = (long long)P.nx + P.ny + P.nz + 1;
+                bool bad = false;
+                for (long long segments = 0; start < iv.upper && segments < limit; ++segments) {
+                    CellPiece s;
+                    cell_piece(P, r, walk, start, s);
+                    if (s.end <= start) {
+                        bad = true;
+                        break;
+                    }
+                    bad |= !finite3d(s.first) || !finite3d(s.second);
+                    if constexpr (POSE) {
+                        double g1[3], g2[3];
+                        const int *lw = s.shared ? s.lower : nullptr;
+                        node_value<T, true>(P, field, s.first, lw, g1);
+                        node_value<T, true>(P, field, s.second, lw, g2);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            og[a] += s.radius * g1[a];
+                            og[a] += s.radius * g2[a];
+                        }
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            dg[a] += s.radius * s.first_position * g1[a];
+                            dg[a] += s.radius * s.second_position * g2[a];
+                        }
+                    }
+                    if constexpr (VOL) {
+                        double vf = factor * s.radius;
+                        if (!isfinite(factor)) vf = product3(seed, r.len, s.radius);
+                        if (s.shared) {
+                            // kernels/projection.py:405-430 (scatter_cell)
+                            double f1[3], f2[3];
+                            const int n[3] = {P.nx, P.ny, P.nz};
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                const double top = (double)(n[a] - 1);
+                                f1[a] = fmin(fmax(s.first[a], 0.0), top) - (double)s.lower[a];
+                                f2[a] = fmin(fmax(s.second[a], 0.0), top) - (double)s.lower[a];
+                            }
+                            const double w1 = inside_support64(P, s.first) ? vf : 0.0;
+                            const double w2 = inside_support64(P, s.second) ? vf : 0.0;
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                double first = w1, second = w2;
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) first *= ((k >> a) & 1) ? f1[a] : 1.0 - f1[a];
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
+                                bad |= !add_increment(acc, corner_of(P, s.lower, k), first + second);
+                            }
+                        } else {
+                            // kernels/projection.py:437-470 (scatter per node)
+#pragma unroll
+                            for (int node = 0; node < 2; ++node) {
+                                const double *pt = node == 0 ? s.first : s.second;
+                                Cell64 c;
+                                if (!cell64(P, pt, c)) continue;
+                                const int lower[3] = {c.lx, c.ly, c.lz};
+                                const double f[3] = {c.fx, c.fy, c.fz};
+#pragma unroll
+                                for (int k = 0; k < 8; ++k) {
+                                    double contribution = vf;
+#pragma unroll
+                                    for (int a = 0; a < 3; ++a) contribution *= ((k >> a) & 1) ? f[a] : 1.0 - f[a];
+                                    bad |= !add_increment(acc, corner_of(P, lower, k), contribution);
+                                }
+                            }
+                        }
+                    }
+                    start = s.end;
+                    advance_cells(P, r, walk, s.end, iv.upper);
+                }
+                bad |= start < iv.upper;
+                if (bad) flag(P, view, DPT_STATUS_RANGE);
+                if constexpr (POSE) {
+                    // kernels/projection.py:1031-1046: endpoint terms of the
+                    // zero-extended support at the clamped clip points
+                    double lo_pt[3], up_pt[3];
+                    const int n[3] = {P.nx, P.ny, P.nz};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double hi = (double)n[a] - 0.5;
+                        lo_pt[a] = fmin(fmax(r.o[a] + iv.lower * r.d[a], -0.5), hi);
+                        up_pt[a] = fmin(fmax(r.o[a] + iv.upper * r.d[a], -0.5), hi);
+                    }
+                    const double lower_value = node_value<T, false>(P, field, lo_pt, nullptr, nullptr);
+                    const double upper_value = node_value<T, false>(P, field, up_pt, nullptr, nullptr);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0, lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
+                        const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0, up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
+                        og[a] += upper_value * up_o - lower_value * lo_o;
+                        dg[a] += upper_value * up_d - lower_value * lo_d;
+                    }
+                    if (isfinite(factor)) {
+                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient);
+                    } else {
+                        double sa[3], da[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            const double sg = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+                            const double rg = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+                            sa[i] = product4(seed, r.len, 1.0, sg);
+                            da[i] = product4(seed, r.len, 1.0, rg);
+                        }
+                        Params unit = P;
+                        for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
+                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (POSE) wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+}
+
 // ------------------------------------------------------- small kernels ----
 
 // Fixed-order FP64 sum of the tile partials of one view (kernels/reductions.py
@@ -1070,6 +1417,7 @@ struct dpt_plan {
     double *d_acc = nullptr;
     float *d_alpha = nullptr;
     unsigned long long *d_count = nullptr;
+    bool cell_gauss = false;  // FP64 exact cell-Gauss integration instead of midpoints
     bool cells = false;       // FP32 sampling gathers from the cell-layout copy
     int mx = 1, my = 1;       // cells along x and y
     int64_t cell_count = 0;
@@ -1205,6 +1553,8 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, b
             hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutPair>), grid, block, 0, s, P, f, o);
         else
             hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutScalar>), grid, block, 0, s, P, f, o);
+    } else if (p->cell_gauss) {
+        hipLaunchKernelGGL((forward_gauss_kernel<T>), grid, block, 0, s, P, f, static_cast<double *>(out));
     } else {
         hipLaunchKernelGGL((forward64_kernel<T>), grid, block, 0, s, P, f, static_cast<double *>(out));
     }
@@ -1242,6 +1592,17 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
             launch_vjp32<T, C, kLayoutPair>(p, P, f, sd, vol, pose, s);
         else
             launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
+    } else if (p->cell_gauss) {
+        const double *sd = static_cast<const double *>(seeds);
+        if (vol && pose)
+            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else if (vol)
+            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, false, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
+        else
+            hipLaunchKernelGGL((vjp_gauss_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+                               p->d_partials);
     } else {
         const double *sd = static_cast<const double *>(seeds);
         const size_t lds = (size_t)P.brick_cap * sizeof(double);
@@ -1350,6 +1711,8 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     if (panels == nullptr && (ray_sources == nullptr || ray_endpoints == nullptr))
         return fail(DPT_E_INVALID, "supply panels or explicit ray sources and endpoints");
     if (panels == nullptr && rows != 1) return fail(DPT_E_INVALID, "explicit rays require rows == 1");
+    if ((needs & DPT_CELL_GAUSS) && arithmetic != DPT_ARITH_FP64)
+        return fail(DPT_E_INVALID, "cell-Gauss integration requires DPT_ARITH_FP64");
     for (int i = 0; i < 9; ++i)
         if (!std::isfinite(grid->object_to_grid[i])) return fail(DPT_E_INVALID, "object_to_grid must be finite");
 
@@ -1359,6 +1722,7 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
     p->arithmetic = arithmetic;
     p->storage = storage;
     p->needs = needs;
+    p->cell_gauss = (needs & DPT_CELL_GAUSS) != 0;
     p->voxels = voxels;
     p->pixels = pixels;
     Params &P = p->base;
@@ -1582,7 +1946,8 @@ int dpt_adjoint_gather(dpt_plan *p, const double *poses, const double *cotangent
                        uint32_t flags, void *stream) {
     if (p == nullptr || poses == nullptr || cotangent == nullptr || out == nullptr)
         return fail(DPT_E_INVALID, "plan, poses, cotangent and out are required");
-    if (p->arithmetic != DPT_ARITH_FP64) return fail(DPT_E_INVALID, "deterministic gather requires strict float64 sampling");
+    if (p->arithmetic != DPT_ARITH_FP64 || p->cell_gauss)
+        return fail(DPT_E_INVALID, "deterministic gather requires strict float64 midpoint sampling");
     if (out_precision != DPT_ARITH_FP32 && out_precision != DPT_ARITH_FP64)
         return fail(DPT_E_INVALID, "out_precision must be DPT_ARITH_FP32 or DPT_ARITH_FP64");
     if (flags & ~(uint32_t)(DPT_ACCUMULATE | DPT_NO_CHECKS))

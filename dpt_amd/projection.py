@@ -26,21 +26,27 @@ from .volumes import GridSpec
 
 CACHE_LIMIT = 8
 PRECISIONS = ("float32", "float64")
+INTEGRATIONS = ("midpoint", "cell_gauss")
 STORAGES = ("float32", "bfloat16", "float16")
 
 
 @dataclass(frozen=True)
 class ProjectionSpec:
-    """Midpoint samples, sample arithmetic and resident field storage.
+    """Midpoint samples, sample arithmetic, resident field storage and the
+    integration rule.
 
     ``precision`` is the sampling and output arithmetic; ``storage`` is how the
     field is held on the device.  Half storage pairs only with FP32 sampling
-    and uses compensated along-ray sums.
+    and uses compensated along-ray sums.  ``integration`` "cell_gauss" is the
+    reference's exact rule (two Gauss nodes per interpolation cell,
+    python/dpt/projection.py:107-135); it requires FP64 sampling and ignores
+    ``samples_per_ray``.
     """
 
     samples_per_ray: int
     precision: str = "float32"
     storage: str = "float32"
+    integration: str = "midpoint"
 
     def __post_init__(self) -> None:
         integer(self.samples_per_ray, "samples_per_ray", minimum=1)
@@ -50,6 +56,10 @@ class ProjectionSpec:
             raise ContractError("storage must be float32, bfloat16 or float16")
         if self.storage != "float32" and self.precision != "float32":
             raise ContractError("half storage requires FP32 sampling")
+        if self.integration not in INTEGRATIONS:
+            raise ContractError("projection integration must be midpoint or cell_gauss")
+        if self.integration == "cell_gauss" and self.precision != "float64":
+            raise ContractError("cell_gauss integration requires float64 sampling")
 
 
 def configuration(grid: GridSpec, geometry, spec: ProjectionSpec) -> str:
@@ -75,6 +85,7 @@ def configuration(grid: GridSpec, geometry, spec: ProjectionSpec) -> str:
             "samples": spec.samples_per_ray,
             "precision": spec.precision,
             "storage": spec.storage,
+            "integration": spec.integration,
         },
         sort_keys=True,
         separators=(",", ":"),
@@ -120,7 +131,8 @@ def decode(value: str) -> Decoded:
                 for p in encoded["panels"]
             )
         )
-    spec = ProjectionSpec(int(encoded["samples"]), encoded["precision"], encoded.get("storage", "float32"))
+    spec = ProjectionSpec(int(encoded["samples"]), encoded["precision"], encoded.get("storage", "float32"),
+                          encoded.get("integration", "midpoint"))
     return Decoded(grid, geometry, spec)
 
 
@@ -187,6 +199,8 @@ def prepare(value: str, device: int, stream: int, needs: int) -> Prepared:
         _check_extent(decoded)
         if gather_cells(decoded):
             needs |= _native.GATHER_CELLS
+        if decoded.spec.integration == "cell_gauss":
+            needs |= _native.CELL_GAUSS
         plan = _native.Plan(
             decoded.grid,
             samples=decoded.spec.samples_per_ray,

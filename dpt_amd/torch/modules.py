@@ -7,10 +7,14 @@ Same constructor and call contract as the reference module
 optional rigid ``anchor``; ``forward(field, pose=None, *, packed_pose=None,
 out=None)`` returns optical depth (views, rows, columns).
 
-One extension: ``storage`` ("float32", "bfloat16", "float16") selects how the
+Extensions: ``storage`` ("float32", "bfloat16", "float16") selects how the
 field is held for sampling.  Half storage rounds the FP32 field once per call
 and samples it with FP32 arithmetic and compensated along-ray sums; gradients
 are returned for the FP32 field.  It requires ``precision="fp32"``.
+``integration="cell_gauss"`` (with ``precision="fp64"``) exposes the
+reference's exact cell-Gauss rule (``ProjectionSpec.integration``,
+python/dpt/projection.py:107-135) through the module; forward and both
+adjoints follow it.
 """
 
 from __future__ import annotations
@@ -30,11 +34,11 @@ from .geometry import compose_pose
 __all__ = ["Projector"]
 
 
-def _projection_spec(precision: str, samples: int, storage: str) -> ProjectionSpec:
+def _projection_spec(precision: str, samples: int, storage: str, integration: str = "midpoint") -> ProjectionSpec:
     if precision not in ("fp32", "fp64"):
         raise ContractError("precision must be fp32 or fp64")
     arithmetic = "float32" if precision == "fp32" else "float64"
-    return ProjectionSpec(samples, arithmetic, storage)
+    return ProjectionSpec(samples, arithmetic, storage, integration)
 
 
 def _geometry(geometry):
@@ -78,6 +82,7 @@ class Projector(torch.nn.Module):
         step_mm: float | None = None,
         anchor: RigidTransform | torch.Tensor | None = None,
         storage: str = "float32",
+        integration: str = "midpoint",
     ) -> None:
         super().__init__()
         if not isinstance(grid, GridSpec):
@@ -88,7 +93,8 @@ class Projector(torch.nn.Module):
         self.storage = storage
         self.samples_per_ray = _samples(grid, samples_per_ray, step_mm)
         self.step_mm = step_mm
-        self._spec = _projection_spec(precision, self.samples_per_ray, storage)
+        self.integration = integration
+        self._spec = _projection_spec(precision, self.samples_per_ray, storage, integration)
         self._configuration = _projection_ops.configuration(self.grid, self.geometry, self._spec)
         self._fixed_poses: dict[str, torch.Tensor] = {}
         if isinstance(anchor, torch.Tensor):

@@ -60,6 +60,12 @@ extern "C" {
  * from a plan-owned cell-layout copy of the field, rebuilt from the linear
  * field at each dpt_forward / dpt_vjp call (the caller's API is unchanged). */
 #define DPT_GATHER_CELLS 64u
+/* dpt_plan_create `needs` option (DPT_ARITH_FP64 only): exact cell-Gauss
+ * integration (kernels/projection.py:522-746, ProjectionSpec.integration =
+ * "cell_gauss", python/dpt/projection.py:107-135) instead of midpoints: the
+ * clipped ray is cut at every interpolation plane and each piece integrated
+ * with two Gauss nodes; `samples` is then unused. */
+#define DPT_CELL_GAUSS 128u
 
 /* dpt_vjp flags */
 #define DPT_VOLUME 1u        /* produce grad_field */

@@ -800,3 +800,341 @@ void ref_gather_f64(const ref_plan *p, const double *poses, const double *cotang
     }
     free(total);
 }
+
+/* ------------------------------------------------------- cell-Gauss --- */
+/*
+ * Exact cell-Gauss integration (kernels/projection.py:522-746): the clipped
+ * ray is cut at every interpolation plane it crosses (integer coordinates),
+ * and each piece is integrated with two-point Gauss nodes, which is exact
+ * for the cubic that a trilinear field is along a line inside one cell.
+ */
+
+typedef struct {
+    int plane[3], step[3];
+    double next[3], inv[3];
+} cell_walk;
+
+/* kernels/projection.py:629-640 (crossing_quotient): the cached reciprocal
+ * with one FMA residual correction; range extremes keep the division. */
+static double crossing_quotient(double delta, double direction, double inverse) {
+    if (inverse == 0.0) return delta / direction;
+    const double q = delta * inverse;
+    if (!isfinite(q) || fabs(q) < 2.2250738585072014e-308) return delta / direction;
+    return fma(fma(-q, direction, delta), inverse, q);
+}
+
+/* kernels/projection.py:643-655 */
+static double cell_crossing(const ref_ray *r, int axis, int plane, const int *shape, double upper, double inverse) {
+    if (r->direction[axis] == 0.0 || plane < 0 || plane >= shape[axis]) return upper;
+    const double c = crossing_quotient((double)plane - r->origin[axis], r->direction[axis], inverse);
+    return c < upper ? c : upper;
+}
+
+/* kernels/projection.py:659-695 (begin_cells) */
+static cell_walk begin_cells(const ref_ray *r, const interval64 *iv, const int *shape) {
+    cell_walk w;
+    memset(&w, 0, sizeof w);
+    for (int a = 0; a < 3; ++a) {
+        const double entry = r->origin[a] + iv->lower * r->direction[a];
+        const double v = r->direction[a];
+        if (v != 0.0) {
+            const double inverse = 1.0 / v;
+            if (isfinite(inverse) && fabs(inverse) >= 2.2250738585072014e-308 && fabs(v) >= 2.2250738585072014e-308)
+                w.inv[a] = inverse;
+        }
+        if (v > 0.0) {
+            w.step[a] = 1;
+            const int c = (int)ceil(entry);
+            w.plane[a] = c > 0 ? c : 0;
+        } else if (v < 0.0) {
+            w.step[a] = -1;
+            const int f = (int)floor(entry);
+            w.plane[a] = f < shape[a] - 1 ? f : shape[a] - 1;
+        }
+        double crossing = cell_crossing(r, a, w.plane[a], shape, iv->upper, w.inv[a]);
+        if (crossing <= iv->lower) {
+            w.plane[a] += w.step[a];
+            crossing = cell_crossing(r, a, w.plane[a], shape, iv->upper, w.inv[a]);
+        }
+        w.next[a] = crossing;
+    }
+    return w;
+}
+
+typedef struct {
+    double end, radius, first_position, second_position;
+    double first[3], second[3];
+    int lower[3];
+    int shared;
+} cell_piece;
+
+/* kernels/projection.py:710-742 (cell_segment) */
+static cell_piece cell_segment(const ref_ray *r, const cell_walk *w, double start, const int *shape) {
+    cell_piece s;
+    const double m = w->next[1] < w->next[2] ? w->next[1] : w->next[2];
+    s.end = w->next[0] < m ? w->next[0] : m;
+    s.radius = (s.end - start) * 0.5;
+    const double centre = start + s.radius;
+    const double offset = s.radius / sqrt(3.0);
+    s.first_position = centre - offset;
+    s.second_position = centre + offset;
+    double mid[3];
+    for (int a = 0; a < 3; ++a) {
+        s.first[a] = r->origin[a] + s.first_position * r->direction[a];
+        s.second[a] = r->origin[a] + s.second_position * r->direction[a];
+        mid[a] = r->origin[a] + centre * r->direction[a];
+    }
+    double unused[3];
+    cell64(mid, shape, s.lower, unused);
+    s.shared = 1;
+    for (int a = 0; a < 3; ++a) {
+        const double lo = (double)s.lower[a], up = lo + 1.0;
+        const double c1 = clampd(s.first[a], 0.0, (double)(shape[a] - 1));
+        const double c2 = clampd(s.second[a], 0.0, (double)(shape[a] - 1));
+        if (c1 < lo || c2 < lo) s.shared = 0;
+        if (s.lower[a] < shape[a] - 2 && (c1 >= up || c2 >= up)) s.shared = 0;
+    }
+    return s;
+}
+
+/* kernels/projection.py:745-755 (advance_cells): every tied plane advances */
+static void advance_cells(cell_walk *w, const ref_ray *r, const int *shape, double end, double upper) {
+    for (int a = 0; a < 3; ++a)
+        if (w->next[a] <= end) {
+            w->plane[a] += w->step[a];
+            w->next[a] = cell_crossing(r, a, w->plane[a], shape, upper, w->inv[a]);
+        }
+}
+
+/* sample_cell with a given cell (kernels/projection.py:278-284) or
+ * sample_field (:288-295) when lower is NULL: value and slope, zero outside
+ * the support. */
+static double sample_point64(const float *field, const double *pt, const int *shape, const int *lower,
+                             double *grad) {
+    if (grad) grad[0] = grad[1] = grad[2] = 0.0;
+    if (!inside64(pt, shape)) return 0.0;
+    int own[3];
+    double f[3], v[8];
+    cell64(pt, shape, own, f);
+    if (lower != NULL)
+        for (int a = 0; a < 3; ++a) f[a] = clampd(pt[a], 0.0, (double)(shape[a] - 1)) - (double)lower[a];
+    load_cell(field, lower != NULL ? lower : own, shape, v);
+    return sample_cell64(v, pt, shape, f, grad);
+}
+
+/* Walk the cells of one clipped ray; returns the number of pieces and sets
+ * BIT_RANGE on a stalled or unfinished walk (kernels/projection.py:714-745). */
+typedef void (*piece_action)(const cell_piece *s, void *ctx);
+
+static void walk_cells(const ref_ray *r, const interval64 *iv, const int *shape, int32_t *status, int view,
+                       piece_action act, void *ctx) {
+    cell_walk w = begin_cells(r, iv, shape);
+    double start = iv->lower;
+    long segments = 0;
+    const long limit = (long)shape[0] + shape[1] + shape[2] + 1;
+    while (start < iv->upper && segments < limit) {
+        cell_piece s = cell_segment(r, &w, start, shape);
+        if (s.end <= start) {
+            status_or(status, view, BIT_RANGE);
+            break;
+        }
+        if (!finite3(s.first) || !finite3(s.second)) status_or(status, view, BIT_RANGE);
+        act(&s, ctx);
+        start = s.end;
+        advance_cells(&w, r, shape, s.end, iv->upper);
+        ++segments;
+    }
+    if (start < iv->upper) status_or(status, view, BIT_RANGE);
+}
+
+typedef struct {
+    const float *field;
+    const int *shape;
+    double integral;
+} forward_ctx;
+
+/* kernels/projection.py:696-711 (_scalar_segment_action) */
+static void forward_piece(const cell_piece *s, void *vctx) {
+    forward_ctx *c = (forward_ctx *)vctx;
+    const int *lower = s->shared ? s->lower : NULL;
+    c->integral += s->radius * sample_point64(c->field, s->first, c->shape, lower, NULL);
+    c->integral += s->radius * sample_point64(c->field, s->second, c->shape, lower, NULL);
+}
+
+/* kernels/projection.py:802-846 with cell_gauss: integral * length */
+void ref_forward_gauss_f64(const ref_plan *p, const float *field, const double *poses, double *out,
+                           int32_t *status) {
+    const int shape[3] = {p->nx, p->ny, p->nz};
+    const int pixels = p->rows * p->cols;
+    const long total_rays = (long)p->views * pixels;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long ray = 0; ray < total_rays; ++ray) {
+        const int view = (int)(ray / pixels), pixel = (int)(ray % pixels);
+        ref_ray r;
+        ray_setup(p, poses + 12 * (size_t)view, view, pixel, &r);
+        if (!valid_ray(&r)) {
+            status_or(status, view, BIT_RANGE);
+            out[ray] = 0.0;
+            continue;
+        }
+        interval64 iv = finite_interval64(r.origin, r.direction, shape);
+        double integral = 0.0;
+        if (iv.upper > iv.lower) {
+            forward_ctx c = {field, shape, 0.0};
+            walk_cells(&r, &iv, shape, status, view, forward_piece, &c);
+            integral = c.integral * r.length;
+        }
+        out[ray] = integral;
+        if (!isfinite(integral)) status_or(status, view, BIT_RANGE);
+    }
+}
+
+typedef struct {
+    const float *field;
+    const int *shape;
+    int32_t *status;
+    int view;
+    double factor, seed, length;
+    double *grad_field;
+    int pose;
+    double og[3], dg[3];
+} vjp_ctx;
+
+/* kernels/projection.py:375-393 / 405-430 with FP32 increments, accumulated
+ * in FP64 */
+static void scatter_point(vjp_ctx *c, const double *pt, double vf) {
+    if (!inside64(pt, c->shape)) return;
+    int lower[3];
+    double f[3];
+    cell64(pt, c->shape, lower, f);
+    for (int k = 0; k < 8; ++k) {
+        double contribution = vf;
+        for (int a = 0; a < 3; ++a) contribution *= ((k >> a) & 1) ? f[a] : 1.0 - f[a];
+        const float rounded = (float)contribution;
+        if (!isfinite(rounded)) status_or(c->status, c->view, BIT_RANGE);
+        add_f64(&c->grad_field[corner_index(lower, c->shape, k)], (double)rounded);
+    }
+}
+
+/* kernels/projection.py:936-1030 (cell-Gauss piece of the VJP) */
+static void vjp_piece(const cell_piece *s, void *vctx) {
+    vjp_ctx *c = (vjp_ctx *)vctx;
+    if (c->pose) {
+        double g1[3], g2[3];
+        const int *lower = s->shared ? s->lower : NULL;
+        sample_point64(c->field, s->first, c->shape, lower, g1);
+        sample_point64(c->field, s->second, c->shape, lower, g2);
+        for (int a = 0; a < 3; ++a) {
+            c->og[a] += s->radius * g1[a];
+            c->og[a] += s->radius * g2[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            c->dg[a] += s->radius * s->first_position * g1[a];
+            c->dg[a] += s->radius * s->second_position * g2[a];
+        }
+    }
+    if (c->grad_field != NULL) {
+        double vf = c->factor * s->radius;
+        if (!isfinite(c->factor)) vf = product3(c->seed, c->length, s->radius);
+        if (s->shared) {
+            /* kernels/projection.py:405-430 (scatter_cell): both nodes' weights
+             * summed in FP64, one FP32 rounding per corner */
+            double f1[3], f2[3];
+            for (int a = 0; a < 3; ++a) {
+                f1[a] = clampd(s->first[a], 0.0, (double)(c->shape[a] - 1)) - (double)s->lower[a];
+                f2[a] = clampd(s->second[a], 0.0, (double)(c->shape[a] - 1)) - (double)s->lower[a];
+            }
+            const double w1 = inside64(s->first, c->shape) ? vf : 0.0;
+            const double w2 = inside64(s->second, c->shape) ? vf : 0.0;
+            for (int k = 0; k < 8; ++k) {
+                double first = w1, second = w2;
+                for (int a = 0; a < 3; ++a) first *= ((k >> a) & 1) ? f1[a] : 1.0 - f1[a];
+                for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
+                const float rounded = (float)(first + second);
+                if (!isfinite(rounded)) status_or(c->status, c->view, BIT_RANGE);
+                add_f64(&c->grad_field[corner_index(s->lower, c->shape, k)], (double)rounded);
+            }
+        } else {
+            scatter_point(c, s->first, vf);
+            scatter_point(c, s->second, vf);
+        }
+    }
+}
+
+/* kernels/projection.py:873-1237 with cell_gauss (FP32 volume destination) */
+void ref_vjp_gauss_f64(const ref_plan *p, const float *field, const double *poses, const double *seeds,
+                       double *grad_field, double *grad_pose, double *per_ray_out, int components,
+                       int32_t *status) {
+    const int shape[3] = {p->nx, p->ny, p->nz};
+    const int pixels = p->rows * p->cols;
+    const long total_rays = (long)p->views * pixels;
+    const int active_pose = grad_pose != NULL || per_ray_out != NULL;
+    double *per_ray = per_ray_out;
+    if (active_pose && per_ray == NULL)
+        per_ray = (double *)calloc((size_t)total_rays * components, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long ray = 0; ray < total_rays; ++ray) {
+        const int view = (int)(ray / pixels), pixel = (int)(ray % pixels);
+        const double *pose = poses + 12 * (size_t)view;
+        double gradient[12] = {0};
+        ref_ray r;
+        ray_setup(p, pose, view, pixel, &r);
+        if (!valid_ray(&r)) {
+            status_or(status, view, BIT_RANGE);
+        } else {
+            interval64 iv = finite_interval64(r.origin, r.direction, shape);
+            double span = iv.upper - iv.lower;
+            const double seed = seeds[ray];
+            if (seed == 0.0) span = 0.0;
+            if (span > 0.0) {
+                vjp_ctx c;
+                memset(&c, 0, sizeof c);
+                c.field = field;
+                c.shape = shape;
+                c.status = status;
+                c.view = view;
+                c.seed = seed;
+                c.length = r.length;
+                c.factor = seed * r.length;
+                c.grad_field = grad_field;
+                c.pose = active_pose;
+                walk_cells(&r, &iv, shape, status, view, vjp_piece, &c);
+                if (active_pose) {
+                    /* kernels/projection.py:1031-1046: endpoint terms of the
+                     * zero-extended support (support_value clamps the clipped
+                     * endpoints into the support) */
+                    double lo_pt[3], up_pt[3];
+                    for (int a = 0; a < 3; ++a) {
+                        lo_pt[a] = clampd(r.origin[a] + iv.lower * r.direction[a], -0.5, (double)shape[a] - 0.5);
+                        up_pt[a] = clampd(r.origin[a] + iv.upper * r.direction[a], -0.5, (double)shape[a] - 0.5);
+                    }
+                    const double lower_value = sample_point64(field, lo_pt, shape, NULL, NULL);
+                    const double upper_value = sample_point64(field, up_pt, shape, NULL, NULL);
+                    for (int a = 0; a < 3; ++a) {
+                        c.og[a] += upper_value * iv.up_o[a] - lower_value * iv.lo_o[a];
+                        c.dg[a] += upper_value * iv.up_d[a] - lower_value * iv.lo_d[a];
+                    }
+                    if (isfinite(c.factor)) {
+                        pose_epilogue(p, pose, &r, c.factor, c.og, c.dg, components, gradient);
+                    } else {
+                        double sa[3], da[3];
+                        for (int i = 0; i < 3; ++i) {
+                            const double sg = p->g[i] * c.og[0] + p->g[3 + i] * c.og[1] + p->g[6 + i] * c.og[2];
+                            const double rg = p->g[i] * c.dg[0] + p->g[3 + i] * c.dg[1] + p->g[6 + i] * c.dg[2];
+                            sa[i] = product4(seed, r.length, 1.0, sg);
+                            da[i] = product4(seed, r.length, 1.0, rg);
+                        }
+                        ref_plan unit = *p;
+                        memset(unit.g, 0, sizeof unit.g);
+                        unit.g[0] = unit.g[4] = unit.g[8] = 1.0;
+                        pose_epilogue(&unit, pose, &r, 1.0, sa, da, components, gradient);
+                    }
+                }
+            }
+        }
+        if (active_pose)
+            for (int k = 0; k < components; ++k) per_ray[(size_t)ray * components + k] = gradient[k];
+    }
+    if (grad_pose != NULL) reduce_pose(p, per_ray, components, grad_pose);
+    if (per_ray != per_ray_out) free(per_ray);
+}

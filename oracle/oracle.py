@@ -80,6 +80,10 @@ def _lib():
         for name in ("ref_vjp_f32", "ref_vjp_f64"):
             getattr(lib, name).argtypes = [ctypes.POINTER(_Plan), p, p, p, p, p, p, ctypes.c_int, p]
             getattr(lib, name).restype = None
+        lib.ref_forward_gauss_f64.argtypes = [ctypes.POINTER(_Plan), p, p, p, p]
+        lib.ref_forward_gauss_f64.restype = None
+        lib.ref_vjp_gauss_f64.argtypes = [ctypes.POINTER(_Plan), p, p, p, p, p, p, ctypes.c_int, p]
+        lib.ref_vjp_gauss_f64.restype = None
         lib.ref_gather_f64.argtypes = [ctypes.POINTER(_Plan), p, p, p, ctypes.c_int, p]
         lib.ref_gather_f64.restype = None
         lib.ref_threads.restype = ctypes.c_int
@@ -194,8 +198,9 @@ class Scene:
             poses = np.broadcast_to(poses, (self.views, 12))
         return np.ascontiguousarray(poses, dtype=np.float64)
 
-    def forward(self, field, poses=None, *, precision: str = "fp32"):
-        """Return (image (V,R,C), status (V,) int32)."""
+    def forward(self, field, poses=None, *, precision: str = "fp32", integration: str = "midpoint"):
+        """Return (image (V,R,C), status (V,) int32).  ``integration="cell_gauss"``
+        (FP64 only) is the exact cell-Gauss rule of kernels/projection.py:522-746."""
         field = np.ascontiguousarray(field, dtype=np.float32).reshape(-1)
         if field.size != self.grid.voxels:
             raise ValueError("field size does not match grid")
@@ -204,6 +209,10 @@ class Scene:
         out = np.zeros(self.image_shape, dtype=dtype)
         status = np.zeros(self.views, dtype=np.int32)
         fn = _lib().ref_forward_f32 if precision == "fp32" else _lib().ref_forward_f64
+        if integration == "cell_gauss":
+            if precision != "fp64":
+                raise ValueError("cell-Gauss integration is FP64 only")
+            fn = _lib().ref_forward_gauss_f64
         fn(ctypes.byref(self.plan), field.ctypes.data, poses.ctypes.data, out.ctypes.data,
            status.ctypes.data)
         return out, status
@@ -219,6 +228,7 @@ class Scene:
         pose: bool = True,
         components: int = 12,
         per_ray: bool = False,
+        integration: str = "midpoint",
     ):
         """Return (grad_field FP64 (voxels,) | None, grad_pose (V,C) | None,
         per_ray (V*R*C, C) | None, status)."""
@@ -233,6 +243,10 @@ class Scene:
         rays = np.zeros((seeds.size, components), dtype=np.float64) if per_ray else None
         status = np.zeros(self.views, dtype=np.int32)
         fn = _lib().ref_vjp_f32 if precision == "fp32" else _lib().ref_vjp_f64
+        if integration == "cell_gauss":
+            if precision != "fp64":
+                raise ValueError("cell-Gauss integration is FP64 only")
+            fn = _lib().ref_vjp_gauss_f64
 
         def ptr(a):
             return None if a is None else a.ctypes.data

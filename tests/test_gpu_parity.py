@@ -31,7 +31,8 @@ def _grid_spec(grid: O.Grid) -> GridSpec:
 class NativeScene:
     """Drive the C ABI directly with torch-owned device buffers."""
 
-    def __init__(self, grid: O.Grid, samples: int, panels=None, rays=None, precision="float32", storage="float32"):
+    def __init__(self, grid: O.Grid, samples: int, panels=None, rays=None, precision="float32", storage="float32",
+                 extra_needs: int = 0):
         self.grid = grid
         if panels is not None:
             packed = np.asarray([p.packed() for p in panels], dtype=np.float64)
@@ -46,7 +47,7 @@ class NativeScene:
         self.storage = storage
         self.plan = _native.Plan(
             _grid_spec(grid), views=views, rows=rows, cols=cols, samples=samples, arithmetic=precision,
-            storage=storage, needs=_native.VOLUME | _native.POSE, device=0, **geo,
+            storage=storage, needs=_native.VOLUME | _native.POSE | extra_needs, device=0, **geo,
         )
         self.stream = torch.cuda.current_stream().cuda_stream
 
