@@ -55,6 +55,7 @@ def _args():
     parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
     parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
     parser.add_argument("--no-gather", action="store_true", help="skip the deterministic adjoint gather measurement")
+    parser.add_argument("--no-cell-gauss", action="store_true", help="skip the exact cell-Gauss measurement")
     parser.add_argument("--gather-only", action="store_true",
                         help="only the deterministic adjoint gather measurement (profiling driver)")
     parser.add_argument("--transport-only", action="store_true",
@@ -420,6 +421,56 @@ def gather_benchmark(grid, geometry, values, target, samples: int) -> dict:
     return result
 
 
+def cell_gauss_benchmark(grid, geometry, values, target, samples: int) -> dict:
+    """Exact cell-Gauss integration of the full scan (FP64 sampling, two Gauss
+    nodes per interpolation cell): forward, forward + pose VJP and forward +
+    joint VJP (second calls), and the quadrature error of the FP64 midpoint
+    rule at ``samples`` against it (relative L2 of the images)."""
+    import torch
+
+    from dpt_amd.torch import Projector
+
+    device = torch.device("cuda:0")
+    exact = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples,
+                      integration="cell_gauss")
+    midpoint = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)
+    field = torch.tensor(values, device=device)
+    leaf = field.clone().requires_grad_(True)
+    pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
+
+    def timed(fn, repeats=2):
+        out, ms = None, []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        return out, ms[-1]
+
+    with torch.no_grad():
+        image, forward_ms = timed(lambda: exact(field, pose.detach()))
+        reference = midpoint(field, pose.detach())
+    cotangent = (image - target.double()).contiguous()
+    _, pose_ms = timed(lambda: torch.autograd.grad(exact(field, pose), pose, cotangent))
+    _, joint_ms = timed(lambda: torch.autograd.grad(exact(leaf, pose), (leaf, pose), cotangent))
+    quadrature = float(torch.linalg.vector_norm(reference - image) / torch.linalg.vector_norm(image))
+    rays = len(geometry.views) * geometry.shape_rows_columns[0] * geometry.shape_rows_columns[1]
+    result = {
+        "forward_ms": round(forward_ms, 3),
+        "forward_pose_vjp_ms": round(pose_ms, 3),
+        "forward_joint_vjp_ms": round(joint_ms, 3),
+        "rays_per_s_forward": round(rays / forward_ms * 1e3, 1),
+        "midpoint_quadrature_rel_l2": quadrature,
+        "midpoint_samples": samples,
+        "note": "FP64 exact rule; the volume VJP adds each piece's 8 FP32-rounded increments with FP64 global "
+                "atomics (no LDS privatisation)",
+    }
+    _log(f"cell-gauss: forward {result['forward_ms']} ms, joint {result['forward_joint_vjp_ms']} ms, "
+         f"midpoint quadrature error {quadrature:.2e}")
+    return result
+
+
 def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -749,6 +800,9 @@ def main() -> int:
     if not args.no_precision:
         _log("precision benchmark")
         result["precision"] = precision_benchmark(grid, geometry, values, target)
+    if not args.no_cell_gauss:
+        _log("exact cell-Gauss integration")
+        result["cell_gauss"] = cell_gauss_benchmark(grid, geometry, values, target, args.samples)
     if not args.no_gather:
         _log("deterministic adjoint gather")
         result["gather"] = gather_benchmark(grid, geometry, values, target, args.samples)
