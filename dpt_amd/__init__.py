@@ -10,7 +10,9 @@ from .contracts import ContractError, NumericalError, TrialDomainError
 from .geometry import (
     BatchedGeometry,
     ConeBeamIntrinsics,
+    CurvedDetectorGeometry,
     DetectorGeometry,
+    ParallelGeometry,
     RayBundleGeometry,
     RigidTransform,
     compose_pose,
@@ -21,10 +23,12 @@ __all__ = [
     "BatchedGeometry",
     "ConeBeamIntrinsics",
     "ContractError",
+    "CurvedDetectorGeometry",
     "DetectorGeometry",
     "GridSpec",
     "ImagingVolume",
     "NumericalError",
+    "ParallelGeometry",
     "RayBundleGeometry",
     "RigidTransform",
     "TrialDomainError",

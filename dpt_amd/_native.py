@@ -20,6 +20,7 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libdpt_amd.so"
 # every symbol include/dpt_amd.h declares
 EXPORTS = (
     "dpt_plan_create",
+    "dpt_plan_create_panels",
     "dpt_plan_destroy",
     "dpt_forward",
     "dpt_vjp",
@@ -171,6 +172,10 @@ def lib():
             ctypes.POINTER(vp),
         ]
         h.dpt_plan_create.restype = ctypes.c_int
+        h.dpt_plan_create_panels.argtypes = [
+            ctypes.POINTER(GridDesc), i32, i32, i32, vp, vp, i32, i32, i32, u32, i32, ctypes.POINTER(vp),
+        ]
+        h.dpt_plan_create_panels.restype = ctypes.c_int
         h.dpt_plan_destroy.argtypes = [vp]
         h.dpt_plan_destroy.restype = ctypes.c_int
         h.dpt_forward.argtypes = [vp, vp, vp, vp, vp]
@@ -242,6 +247,8 @@ class Plan:
         storage: str,
         needs: int,
         device: int,
+        panel_records: np.ndarray | None = None,
+        panel_modes: np.ndarray | None = None,
     ) -> None:
         h = lib()
         desc = GridDesc()
@@ -252,6 +259,22 @@ class Plan:
             desc.grid_origin[i] = grid.origin_mm[i]
         self._keep = []
         panel_ptr = src_ptr = end_ptr = None
+        if panel_records is not None:
+            records = np.ascontiguousarray(panel_records, dtype=np.float64).reshape(views, 18)
+            modes = np.ascontiguousarray(panel_modes, dtype=np.int32).reshape(views)
+            self._keep += [records, modes]
+            handle = ctypes.c_void_p()
+            rc = h.dpt_plan_create_panels(
+                ctypes.byref(desc), views, rows, cols, records.ctypes.data, modes.ctypes.data, samples,
+                ARITH_CODES[arithmetic], STORAGE_CODES[storage], needs, device, ctypes.byref(handle),
+            )
+            check(rc, "dpt_plan_create_panels")
+            self.handle = handle
+            self.views, self.rows, self.cols = views, rows, cols
+            self.arithmetic, self.storage = arithmetic, storage
+            self.device = device
+            self._codes = np.zeros(views, dtype=np.int32)
+            return
         if panels is not None:
             arr = np.ascontiguousarray(panels, dtype=np.float64)
             self._keep.append(arr)

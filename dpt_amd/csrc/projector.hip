@@ -1398,6 +1398,41 @@ __global__ __launch_bounds__(kBlock) void count_hits_kernel(Params P, unsigned l
     if ((threadIdx.x & 63) == 0 && lanes) atomicAdd(count, lanes);
 }
 
+// Expand per-view panel records into world segments, one per pixel
+// (kernels/batched_geometry.py:9-39): record = source (the beam direction
+// for parallel panels), pixel (0,0), u, v, column and row pitch, length
+// (parallel source distance, or the cylinder radius), radial vector; mode 0
+// flat cone, 1 parallel, 2 cylindrical.
+__global__ __launch_bounds__(kBlock) void panel_rays_kernel(const double *__restrict__ meta,
+                                                            const int *__restrict__ modes, int64_t per_view,
+                                                            int cols, int64_t total, double *__restrict__ src,
+                                                            double *__restrict__ end) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+        const int view = (int)(i / per_view);
+        const int pixel = (int)(i - (int64_t)view * per_view);
+        const int row = pixel / cols, col = pixel - row * cols;
+        const double *m = meta + 18 * (size_t)view;
+        const double du = m[12], dv = m[13], length = m[14];
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = m[k];
+            b[k] = m[3 + k] + (double)col * du * m[6 + k] + (double)row * dv * m[9 + k];
+        }
+        if (modes[view] == 1) {
+            for (int k = 0; k < 3; ++k) a[k] = b[k] - length * a[k];
+        } else if (modes[view] == 2) {
+            const double angle = (double)col * du / length;
+            const double c = cos(angle) - 1.0, sn = sin(angle);
+            for (int k = 0; k < 3; ++k)
+                b[k] = m[3 + k] + c * m[15 + k] + sn * length * m[6 + k] + (double)row * dv * m[9 + k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            src[3 * i + k] = a[k];
+            end[3 * i + k] = b[k];
+        }
+    }
+}
+
 }  // namespace dpt
 
 // ================================================================ C ABI ====
@@ -1687,10 +1722,37 @@ int dpt_plan_destroy(dpt_plan *p) {
     return DPT_OK;
 }
 
-int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols, const double *panels,
-                    const double *ray_sources, const double *ray_endpoints, int32_t samples, int32_t arithmetic,
-                    int32_t storage, uint32_t needs, int32_t device, dpt_plan **out) {
+}  // extern "C"
+
+namespace {
+
+// Shared body of dpt_plan_create (panels or host rays) and
+// dpt_plan_create_panels (panel records expanded into rays on the device;
+// the plan is then a bundle of rows*cols rays per view).
+int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols, const double *panels,
+                const double *ray_sources, const double *ray_endpoints, const double *panel_meta,
+                const int32_t *panel_modes, int32_t samples, int32_t arithmetic, int32_t storage, uint32_t needs,
+                int32_t device, dpt_plan **out) {
     if (out == nullptr || grid == nullptr) return fail(DPT_E_INVALID, "grid and out are required");
+    const bool generated = panel_meta != nullptr;
+    int image_cols = cols;  // columns of one detector row (the bundle below is rows*cols rays per view)
+    if (generated) {
+        if (panel_modes == nullptr || panels != nullptr || ray_sources != nullptr || ray_endpoints != nullptr)
+            return fail(DPT_E_INVALID, "panel records take modes and no panels or rays");
+        if (views < 1 || rows < 1 || cols < 1) return fail(DPT_E_INVALID, "views, rows and cols must be positive");
+        if ((int64_t)rows * cols > INT32_MAX) return fail(DPT_E_INVALID, "image exceeds signed 32-bit indexing");
+        for (int v = 0; v < views; ++v) {
+            if (panel_modes[v] < 0 || panel_modes[v] > 2) return fail(DPT_E_INVALID, "panel mode must be 0, 1 or 2");
+            for (int k = 0; k < 18; ++k)
+                if (!std::isfinite(panel_meta[18 * (size_t)v + k]))
+                    return fail(DPT_E_INVALID, "panel records must be finite");
+            if (panel_modes[v] != 0 && !(panel_meta[18 * (size_t)v + 14] > 0.0))
+                return fail(DPT_E_INVALID, "parallel length / cylinder radius must be positive");
+        }
+        image_cols = cols;
+        cols = rows * cols;
+        rows = 1;
+    }
     *out = nullptr;
     if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
         return fail(DPT_E_INVALID, "grid counts must be positive");
@@ -1708,7 +1770,7 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         return fail(DPT_E_INVALID, "storage must be DPT_STORE_F32, DPT_STORE_BF16 or DPT_STORE_F16");
     if (arithmetic == DPT_ARITH_FP32 && (grid->nx > (1 << 24) || grid->ny > (1 << 24) || grid->nz > (1 << 24)))
         return fail(DPT_E_INVALID, "FP32 sampling requires grid axis counts at most 2**24");
-    if (panels == nullptr && (ray_sources == nullptr || ray_endpoints == nullptr))
+    if (!generated && panels == nullptr && (ray_sources == nullptr || ray_endpoints == nullptr))
         return fail(DPT_E_INVALID, "supply panels or explicit ray sources and endpoints");
     if (panels == nullptr && rows != 1) return fail(DPT_E_INVALID, "explicit rays require rows == 1");
     if ((needs & DPT_CELL_GAUSS) && arithmetic != DPT_ARITH_FP64)
@@ -1799,9 +1861,28 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
         const size_t bytes = sizeof(double) * 3 * (size_t)pixels;
         ALLOC(p->d_ray_src, bytes);
         ALLOC(p->d_ray_end, bytes);
-        if (hipMemcpy(p->d_ray_src, ray_sources, bytes, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(p->d_ray_end, ray_endpoints, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        if (generated) {
+            double *d_meta = nullptr;
+            int *d_modes = nullptr;
+            ALLOC(d_meta, sizeof(double) * 18 * views);
+            if (hipMalloc((void **)&d_modes, sizeof(int) * views) != hipSuccess) {
+                (void)hipFree(d_meta);
+                return cleanup(DPT_E_MEMORY, "hipMalloc failed for the panel modes");
+            }
+            bool ok = hipMemcpy(d_meta, panel_meta, sizeof(double) * 18 * views, hipMemcpyHostToDevice) == hipSuccess &&
+                      hipMemcpy(d_modes, panel_modes, sizeof(int) * views, hipMemcpyHostToDevice) == hipSuccess;
+            if (ok) {
+                hipLaunchKernelGGL(panel_rays_kernel, dim3(grid_blocks(pixels)), dim3(kBlock), 0, 0, d_meta, d_modes,
+                                   (int64_t)cols, image_cols, pixels, p->d_ray_src, p->d_ray_end);
+                ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+            }
+            (void)hipFree(d_meta);
+            (void)hipFree(d_modes);
+            if (!ok) return cleanup(DPT_E_DEVICE, "panel ray generation failed");
+        } else if (hipMemcpy(p->d_ray_src, ray_sources, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(p->d_ray_end, ray_endpoints, bytes, hipMemcpyHostToDevice) != hipSuccess) {
             return cleanup(DPT_E_DEVICE, "copy of rays failed");
+        }
     }
     P.panels = p->d_panels;
     P.ray_src = p->d_ray_src;
@@ -1828,6 +1909,25 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
 #undef ALLOC
     *out = p;
     return DPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols, const double *panels,
+                    const double *ray_sources, const double *ray_endpoints, int32_t samples, int32_t arithmetic,
+                    int32_t storage, uint32_t needs, int32_t device, dpt_plan **out) {
+    return plan_create(grid, views, rows, cols, panels, ray_sources, ray_endpoints, nullptr, nullptr, samples,
+                       arithmetic, storage, needs, device, out);
+}
+
+int dpt_plan_create_panels(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols,
+                           const double *panel_records, const int32_t *panel_modes, int32_t samples,
+                           int32_t arithmetic, int32_t storage, uint32_t needs, int32_t device, dpt_plan **out) {
+    if (panel_records == nullptr) return fail(DPT_E_INVALID, "panel records are required");
+    return plan_create(grid, views, rows, cols, nullptr, nullptr, nullptr, panel_records, panel_modes, samples,
+                       arithmetic, storage, needs, device, out);
 }
 
 int dpt_forward(dpt_plan *p, const void *field, const double *poses, void *out, void *stream) {

@@ -143,6 +143,10 @@ class DetectorGeometry:
             tuple(self.origin_mm[i] + column * du * self.u[i] + row * dv * self.v[i] for i in range(3)),
         )
 
+    def record(self) -> tuple[tuple[float, ...], int]:
+        """Panel record for device ray generation (kernels/batched_geometry.py:9-39): mode 0."""
+        return (*self.source_mm, *self.origin_mm, *self.u, *self.v, *self.spacing_mm, 0.0, 0.0, 0.0, 0.0), 0
+
     def packed(self) -> tuple[float, ...]:
         """source, pixel (0,0) centre, column step u*du, row step v*dv (projection.py:594-598)."""
         du, dv = self.spacing_mm
@@ -152,6 +156,129 @@ class DetectorGeometry:
             *(x * du for x in self.u),
             *(x * dv for x in self.v),
         )
+
+
+def _pixel_index(row, column, shape) -> None:
+    if type(row) is not int or type(column) is not int:
+        raise ContractError("pixel indices must be integers")
+    if not (0 <= row < shape[0] and 0 <= column < shape[1]):
+        raise ContractError("pixel index outside detector")
+
+
+@dataclass(frozen=True)
+class ParallelGeometry:
+    """Finite parallel segments ``s[r,c] = p[r,c] - length * direction``
+    (geometry.py:306-366): ``origin_mm`` is the endpoint of pixel (0,0), ``u``
+    increases columns, ``v`` rows; ``shape`` is (rows, columns) and
+    ``spacing_mm`` (column, row), all in millimetres."""
+
+    origin_mm: Vector3
+    direction: Vector3
+    length_mm: float
+    u: Vector3
+    v: Vector3
+    spacing_mm: tuple[float, float]
+    shape: tuple[int, int]
+
+    def __post_init__(self) -> None:
+        direction = vector3(self.direction, "parallel direction")
+        if abs(dot(direction, direction) - 1.0) > 1e-10:
+            raise ContractError("parallel direction must have unit length")
+        length = finite_tuple((self.length_mm,), "parallel length", positive=True)[0]
+        origin = vector3(self.origin_mm, "parallel origin")
+        source = tuple(origin[i] - length * direction[i] for i in range(3))
+        panel = DetectorGeometry(source, origin, self.u, self.v, self.spacing_mm, self.shape)
+        object.__setattr__(self, "origin_mm", origin)
+        object.__setattr__(self, "direction", direction)
+        object.__setattr__(self, "length_mm", length)
+        for name in ("u", "v", "spacing_mm", "shape"):
+            object.__setattr__(self, name, getattr(panel, name))
+
+    @property
+    def pixels(self) -> int:
+        return self.shape[0] * self.shape[1]
+
+    def pixel_centre(self, row: int, column: int) -> Vector3:
+        _pixel_index(row, column, self.shape)
+        du, dv = self.spacing_mm
+        return cast(
+            Vector3,
+            tuple(self.origin_mm[i] + column * du * self.u[i] + row * dv * self.v[i] for i in range(3)),
+        )
+
+    def segment(self, row: int, column: int) -> tuple[Vector3, Vector3]:
+        """Source and endpoint of pixel (row, column)."""
+        end = self.pixel_centre(row, column)
+        return cast(Vector3, tuple(end[i] - self.length_mm * self.direction[i] for i in range(3))), end
+
+    def record(self) -> tuple[tuple[float, ...], int]:
+        return (*self.direction, *self.origin_mm, *self.u, *self.v, *self.spacing_mm, self.length_mm, 0.0, 0.0,
+                0.0), 1
+
+
+@dataclass(frozen=True)
+class CurvedDetectorGeometry:
+    """Cylindrical pixels ``p = axis + radius cos(theta) + u R sin(theta)``
+    (geometry.py:369-431): the cylinder axis passes through ``source_mm``
+    along ``v``; pixel (0,0) is ``origin_mm``; columns follow the tangent
+    ``u`` with arc length ``spacing_mm[0]``, rows move along ``v`` by
+    ``spacing_mm[1]``."""
+
+    source_mm: Vector3
+    origin_mm: Vector3
+    u: Vector3
+    v: Vector3
+    spacing_mm: tuple[float, float]
+    shape: tuple[int, int]
+
+    def __post_init__(self) -> None:
+        panel = DetectorGeometry(self.source_mm, self.origin_mm, self.u, self.v, self.spacing_mm, self.shape)
+        for name in ("source_mm", "origin_mm", "u", "v", "spacing_mm", "shape"):
+            object.__setattr__(self, name, getattr(panel, name))
+        radial = self.radial_mm
+        if not math.isfinite(self.radius_mm) or self.radius_mm <= 0.0:
+            raise ContractError("curved detector radius must be finite and positive")
+        if abs(dot(radial, self.u)) > 1e-10 * self.radius_mm:
+            raise ContractError("curved detector u must be tangent to its source-centred cylinder")
+        if self.spacing_mm[0] * (self.shape[1] - 1) >= 2 * math.pi * self.radius_mm:
+            raise ContractError("curved detector columns must cover less than one full revolution")
+
+    @property
+    def radial_mm(self) -> Vector3:
+        offset = tuple(self.origin_mm[i] - self.source_mm[i] for i in range(3))
+        along = dot(offset, self.v)
+        return cast(Vector3, tuple(offset[i] - along * self.v[i] for i in range(3)))
+
+    @property
+    def radius_mm(self) -> float:
+        radial = self.radial_mm
+        return math.sqrt(dot(radial, radial))
+
+    @property
+    def pixels(self) -> int:
+        return self.shape[0] * self.shape[1]
+
+    def pixel_centre(self, row: int, column: int) -> Vector3:
+        _pixel_index(row, column, self.shape)
+        radial, radius = self.radial_mm, self.radius_mm
+        angle = column * self.spacing_mm[0] / radius
+        return cast(
+            Vector3,
+            tuple(
+                self.origin_mm[i]
+                + (math.cos(angle) - 1.0) * radial[i]
+                + math.sin(angle) * radius * self.u[i]
+                + row * self.spacing_mm[1] * self.v[i]
+                for i in range(3)
+            ),
+        )
+
+    def record(self) -> tuple[tuple[float, ...], int]:
+        return (*self.source_mm, *self.origin_mm, *self.u, *self.v, *self.spacing_mm, self.radius_mm,
+                *self.radial_mm), 2
+
+
+PANEL_TYPES = (DetectorGeometry, ParallelGeometry, CurvedDetectorGeometry)
 
 
 @dataclass(frozen=True)
@@ -190,17 +317,19 @@ class RayBundleGeometry:
 
 @dataclass(frozen=True)
 class BatchedGeometry:
-    """O(views) panel metadata for an image of shape (views, rows, columns)."""
+    """O(views) panel metadata for an image of shape (views, rows, columns);
+    flat cone, parallel and cylindrical panels may share one batch
+    (geometry.py:462-520)."""
 
-    views: tuple[DetectorGeometry, ...]
+    views: tuple[DetectorGeometry | ParallelGeometry | CurvedDetectorGeometry, ...]
 
     def __post_init__(self) -> None:
         if (
             not isinstance(self.views, tuple)
             or not self.views
-            or any(not isinstance(view, DetectorGeometry) for view in self.views)
+            or any(not isinstance(view, PANEL_TYPES) for view in self.views)
         ):
-            raise ContractError("views must be a non-empty immutable tuple of cone panels")
+            raise ContractError("views must be a non-empty immutable tuple of panel geometries")
         if any(view.shape != self.views[0].shape for view in self.views):
             raise ContractError("batched views must share (rows,columns)")
         if self.pixels > 2**31 - 1:
@@ -217,6 +346,20 @@ class BatchedGeometry:
     @property
     def pixels(self) -> int:
         return len(self.views) * self.views[0].pixels
+
+    @property
+    def flat_cone(self) -> bool:
+        """Every view is a flat cone panel (the projected-window kernels apply)."""
+        return all(type(view) is DetectorGeometry for view in self.views)
+
+    def segment(self, view: int, row: int, column: int) -> tuple[Vector3, Vector3]:
+        """One finite world segment (geometry.py:503-511)."""
+        if type(view) is not int or not 0 <= view < len(self.views):
+            raise ContractError("view index must be inside the batch")
+        panel = self.views[view]
+        if isinstance(panel, ParallelGeometry):
+            return panel.segment(row, column)
+        return panel.source_mm, panel.pixel_centre(row, column)
 
 
 def _panel_shape(shape) -> None:

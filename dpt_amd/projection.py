@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _native
 from .contracts import ContractError, integer, raise_view_statuses
-from .geometry import BatchedGeometry, DetectorGeometry, RayBundleGeometry
+from .geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry, RayBundleGeometry
 from .volumes import GridSpec
 
 CACHE_LIMIT = 8
@@ -62,22 +62,30 @@ class ProjectionSpec:
             raise ContractError("cell_gauss integration requires float64 sampling")
 
 
+def _encode_panel(p) -> dict:
+    common = {"origin_mm": list(p.origin_mm), "u": list(p.u), "v": list(p.v), "spacing_mm": list(p.spacing_mm),
+              "shape": list(p.shape)}
+    if isinstance(p, ParallelGeometry):
+        return {"kind": "parallel", "direction": list(p.direction), "length_mm": p.length_mm, **common}
+    kind = "curved" if isinstance(p, CurvedDetectorGeometry) else "cone"
+    return {"kind": kind, "source_mm": list(p.source_mm), **common}
+
+
+def _decode_panel(p: dict):
+    common = (tuple(p["u"]), tuple(p["v"]), tuple(p["spacing_mm"]), tuple(p["shape"]))
+    kind = p.get("kind", "cone")
+    if kind == "parallel":
+        return ParallelGeometry(tuple(p["origin_mm"]), tuple(p["direction"]), p["length_mm"], *common)
+    cls = CurvedDetectorGeometry if kind == "curved" else DetectorGeometry
+    return cls(tuple(p["source_mm"]), tuple(p["origin_mm"]), *common)
+
+
 def configuration(grid: GridSpec, geometry, spec: ProjectionSpec) -> str:
     """Encode fixed physical values as a JSON string (dispatcher-safe)."""
     if isinstance(geometry, RayBundleGeometry):
         views = {"rays": [list(map(list, geometry.sources_mm)), list(map(list, geometry.endpoints_mm))]}
     else:
-        views = {"panels": [
-            {
-                "source_mm": list(p.source_mm),
-                "origin_mm": list(p.origin_mm),
-                "u": list(p.u),
-                "v": list(p.v),
-                "spacing_mm": list(p.spacing_mm),
-                "shape": list(p.shape),
-            }
-            for p in geometry.views
-        ]}
+        views = {"panels": [_encode_panel(p) for p in geometry.views]}
     return json.dumps(
         {
             "grid": grid.encode(),
@@ -118,19 +126,7 @@ def decode(value: str) -> Decoded:
         sources, endpoints = encoded["rays"]
         geometry = RayBundleGeometry(tuple(map(tuple, sources)), tuple(map(tuple, endpoints)))
     else:
-        geometry = BatchedGeometry(
-            tuple(
-                DetectorGeometry(
-                    tuple(p["source_mm"]),
-                    tuple(p["origin_mm"]),
-                    tuple(p["u"]),
-                    tuple(p["v"]),
-                    tuple(p["spacing_mm"]),
-                    tuple(p["shape"]),
-                )
-                for p in encoded["panels"]
-            )
-        )
+        geometry = BatchedGeometry(tuple(_decode_panel(p) for p in encoded["panels"]))
     spec = ProjectionSpec(int(encoded["samples"]), encoded["precision"], encoded.get("storage", "float32"),
                           encoded.get("integration", "midpoint"))
     return Decoded(grid, geometry, spec)
@@ -154,8 +150,14 @@ def _native_geometry(decoded: Decoded):
         src = np.asarray(geometry.sources_mm, dtype=np.float64)
         end = np.asarray(geometry.endpoints_mm, dtype=np.float64)
         return dict(views=1, rows=1, cols=geometry.rays, panels=None, rays=(src, end))
-    panels = np.asarray([p.packed() for p in geometry.views], dtype=np.float64)
     rows, cols = geometry.shape_rows_columns
+    if not geometry.flat_cone:
+        # parallel / cylindrical panels: one world segment per pixel, expanded on the device
+        records = [p.record() for p in geometry.views]
+        return dict(views=len(geometry.views), rows=rows, cols=cols, panels=None, rays=None,
+                    panel_records=np.asarray([r for r, _ in records], dtype=np.float64),
+                    panel_modes=np.asarray([m for _, m in records], dtype=np.int32))
+    panels = np.asarray([p.packed() for p in geometry.views], dtype=np.float64)
     return dict(views=len(geometry.views), rows=rows, cols=cols, panels=panels, rays=None)
 
 
@@ -164,11 +166,11 @@ def _check_extent(decoded: Decoded) -> None:
     geometry = decoded.geometry
     if isinstance(geometry, RayBundleGeometry):
         return
-    for panel in geometry.views:
+    for view, panel in enumerate(geometry.views):
         for row in (0, panel.shape[0] - 1):
             for col in (0, panel.shape[1] - 1):
-                end = panel.pixel_centre(row, col)
-                if not all(math.isfinite(x) for x in end) or not math.isfinite(math.dist(end, panel.source_mm)):
+                source, end = geometry.segment(view, row, col)
+                if not all(math.isfinite(x) for x in (*source, *end)) or not math.isfinite(math.dist(end, source)):
                     raise ContractError("detector extent exceeds the FP64 finite-ray range")
 
 

@@ -25,7 +25,7 @@ from collections.abc import Sequence
 import torch
 
 from ..contracts import ContractError, finite_scalar, integer
-from ..geometry import BatchedGeometry, DetectorGeometry, RayBundleGeometry, RigidTransform
+from ..geometry import PANEL_TYPES, BatchedGeometry, RayBundleGeometry, RigidTransform
 from ..projection import ProjectionSpec
 from ..volumes import GridSpec
 from . import _projection_ops
@@ -44,7 +44,7 @@ def _projection_spec(precision: str, samples: int, storage: str, integration: st
 def _geometry(geometry):
     if isinstance(geometry, (BatchedGeometry, RayBundleGeometry)):
         return geometry
-    if isinstance(geometry, DetectorGeometry):
+    if isinstance(geometry, PANEL_TYPES):
         return BatchedGeometry((geometry,))
     if isinstance(geometry, Sequence):
         return BatchedGeometry(tuple(geometry))

@@ -109,6 +109,29 @@ int dpt_plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int3
                     int32_t samples, int32_t arithmetic, int32_t storage, uint32_t needs,
                     int32_t device, dpt_plan **out);
 
+/*
+ * Prepare a multiview plan from per-view panel records, expanded into one
+ * world segment per pixel on the device (flat cone, parallel and cylindrical
+ * panels may share one batch).  Replaces prepare_geometry
+ * (python/dpt/geometry_device.py:190-233) with the generate kernel of
+ * kernels/batched_geometry.py:9-39, for BatchedGeometry views of
+ * DetectorGeometry / ParallelGeometry / CurvedDetectorGeometry
+ * (python/dpt/geometry.py:101-460).
+ * panel_records: host float64 (views, 18) = source (for parallel panels the
+ *   unit beam direction), pixel (0,0) endpoint, u, v, column pitch, row
+ *   pitch, length (parallel: source distance behind each pixel; cylindrical:
+ *   radius), radial vector (cylindrical: pixel (0,0) minus source, with its
+ *   v component removed).
+ * panel_modes: host int32 (views): 0 flat cone, 1 parallel, 2 cylindrical.
+ * The image is (views, rows, cols) as for dpt_plan_create; other arguments
+ * as there.  Plans made here gather with every pixel of a view in the
+ * adjoint gather (no projected-window culling).
+ */
+int dpt_plan_create_panels(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t cols,
+                           const double *panel_records, const int32_t *panel_modes, int32_t samples,
+                           int32_t arithmetic, int32_t storage, uint32_t needs, int32_t device,
+                           dpt_plan **out);
+
 /* Release every device and pinned buffer of the plan. */
 int dpt_plan_destroy(dpt_plan *plan);
 

@@ -152,7 +152,8 @@ class Panel:
 
 
 class Scene:
-    """A grid plus either panels (one per view) or explicit rays (one view)."""
+    """A grid plus either panels (one per view) or explicit rays ((rays, 3) for
+    one view, (views, rays, 3) for one bundle per view)."""
 
     def __init__(
         self,
@@ -179,9 +180,11 @@ class Scene:
             plan.panels = self._panels.ctypes.data
         else:
             sources, endpoints = rays
+            sources = np.asarray(sources, dtype=np.float64)
+            views = sources.shape[0] if sources.ndim == 3 else 1  # (views, rays, 3): one bundle per view
             self._sources = np.ascontiguousarray(sources, dtype=np.float64).reshape(-1, 3)
             self._endpoints = np.ascontiguousarray(endpoints, dtype=np.float64).reshape(-1, 3)
-            plan.views, plan.rows, plan.cols = 1, 1, self._sources.shape[0]
+            plan.views, plan.rows, plan.cols = views, 1, self._sources.shape[0] // views
             plan.bundle = 1
             plan.ray_sources = self._sources.ctypes.data
             plan.ray_endpoints = self._endpoints.ctypes.data
@@ -288,8 +291,9 @@ class Scene:
             pose = [float(x) for x in poses[view]]
             for pixel in range(self.rows * self.cols):
                 if self.plan.bundle:
-                    src = [float(x) for x in self._sources[pixel]]
-                    end = [float(x) for x in self._endpoints[pixel]]
+                    index = view * self.rows * self.cols + pixel
+                    src = [float(x) for x in self._sources[index]]
+                    end = [float(x) for x in self._endpoints[index]]
                 else:
                     pn = [float(x) for x in self._panels[view]]
                     row, col = divmod(pixel, self.cols)

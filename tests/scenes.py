@@ -141,3 +141,52 @@ def gather_scene(case: str):
         (-8.0, 0.17, 0.23), (8.0, -0.43, -0.37), (math.sin(0.1), math.cos(0.1), 0.0), (0.0, 0.0, 1.0), (0.19, 0.23), (2, 5)
     )
     return grid, dict(panels=[panel]), pose
+
+
+def mixed_panels(grid: O.Grid, rows: int = 9, cols: int = 11):
+    """A cone panel, a parallel panel and a cylindrical panel of one shape,
+    looking at the grid from three directions (geometry.py:306-431)."""
+    nx, ny, nz = grid.xyz
+    centre = [grid.origin[i] + ((grid.xyz[i] - 1) / 2) * grid.spacing[i] for i in range(3)]
+    extent = max(nx * grid.spacing[0], ny * grid.spacing[1], nz * grid.spacing[2])
+    pitch = 1.3 * extent / max(rows, cols)
+    cone = O.Panel((centre[0], centre[1] - 60.0, centre[2]),
+                   (centre[0] - 0.5 * (cols - 1) * pitch, centre[1] + 40.0, centre[2] - 0.5 * (rows - 1) * pitch),
+                   (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (pitch, pitch), (rows, cols))
+    a = 0.35
+    direction = (math.cos(a), math.sin(a), 0.0)  # unit beam direction
+    u = (-math.sin(a), math.cos(a), 0.0)
+    origin = tuple(centre[i] + 30.0 * direction[i] - 0.5 * (cols - 1) * 0.8 * pitch * u[i]
+                   - (0.5 * (rows - 1) * 0.8 * pitch if i == 2 else 0.0) for i in range(3))
+    parallel = dict(origin_mm=origin, direction=direction, length_mm=70.0, u=u, v=(0.0, 0.0, 1.0),
+                    spacing_mm=(0.8 * pitch, 0.8 * pitch), shape=(rows, cols))
+    # cylinder of radius 90 about the vertical axis through the source
+    source = (centre[0] - 50.0, centre[1] + 10.0, centre[2])
+    b = -0.25  # pixel (0,0) sits at this angle from the +x direction around the axis
+    radius = 90.0
+    origin_c = (source[0] + radius * math.cos(b), source[1] + radius * math.sin(b), centre[2] - 0.5 * (rows - 1) * pitch)
+    u_c = (-math.sin(b), math.cos(b), 0.0)
+    curved = dict(source_mm=source, origin_mm=origin_c, u=u_c, v=(0.0, 0.0, 1.0),
+                  spacing_mm=(pitch * 1.4, pitch), shape=(rows, cols))
+    return cone, parallel, curved
+
+
+def parallel_segment(p: dict, row: int, col: int):
+    """geometry.py:349-354 restated: endpoint = origin + col du u + row dv v,
+    source = endpoint - length * direction."""
+    du, dv = p["spacing_mm"]
+    end = tuple(p["origin_mm"][i] + col * du * p["u"][i] + row * dv * p["v"][i] for i in range(3))
+    return tuple(end[i] - p["length_mm"] * p["direction"][i] for i in range(3)), end
+
+
+def curved_segment(p: dict, row: int, col: int):
+    """geometry.py:408-431 restated: the pixel rotates about the axis through
+    the source along v by the arc angle col * du / radius."""
+    offset = [p["origin_mm"][i] - p["source_mm"][i] for i in range(3)]
+    along = sum(offset[i] * p["v"][i] for i in range(3))
+    radial = [offset[i] - along * p["v"][i] for i in range(3)]
+    radius = math.sqrt(sum(x * x for x in radial))
+    angle = col * p["spacing_mm"][0] / radius
+    end = tuple(p["origin_mm"][i] + (math.cos(angle) - 1.0) * radial[i] + math.sin(angle) * radius * p["u"][i]
+                + row * p["spacing_mm"][1] * p["v"][i] for i in range(3))
+    return tuple(p["source_mm"]), end

@@ -274,3 +274,43 @@ def test_batched_geometry_requires_equal_shapes():
     b = ConeBeamIntrinsics(100.0, 150.0, (4, 5), (1.0, 1.0)).view(0.0)
     with pytest.raises(ContractError, match="share"):
         BatchedGeometry((a, b))
+
+
+def test_parallel_and_cylindrical_panels_validate_and_round_trip():
+    """geometry.py:306-431 contracts, and the JSON configuration keeps each
+    view's kind."""
+    import math
+
+    from dpt_amd.geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry
+    from dpt_amd.projection import ProjectionSpec, configuration, decode
+    from dpt_amd.volumes import GridSpec
+
+    parallel = ParallelGeometry((5.0, 0.0, 0.0), (1.0, 0.0, 0.0), 10.0, (0.0, 1.0, 0.0), (0.0, 0.0, 1.0),
+                                (0.5, 0.5), (3, 4))
+    source, end = parallel.segment(1, 2)
+    assert end == (5.0, 1.0, 0.5) and source == (-5.0, 1.0, 0.5)
+    with pytest.raises(ContractError, match="unit length"):
+        ParallelGeometry((5.0, 0.0, 0.0), (2.0, 0.0, 0.0), 10.0, (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (0.5, 0.5), (3, 4))
+    curved = CurvedDetectorGeometry((0.0, 0.0, 0.0), (100.0, 0.0, -1.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0),
+                                    (2.0, 1.0), (3, 5))
+    assert math.isclose(curved.radius_mm, 100.0)
+    for col in range(5):
+        p = curved.pixel_centre(2, col)
+        assert math.isclose(math.hypot(p[0], p[1]), 100.0, rel_tol=1e-14)
+        assert math.isclose(math.atan2(p[1], p[0]), col * 2.0 / 100.0, rel_tol=1e-12, abs_tol=1e-15)
+        assert p[2] == 1.0
+    with pytest.raises(ContractError, match="tangent"):
+        CurvedDetectorGeometry((0.0, 0.0, 0.0), (100.0, 0.0, 0.0), (0.6, 0.8, 0.0), (0.0, 0.0, 1.0), (2.0, 1.0), (3, 5))
+    with pytest.raises(ContractError, match="revolution"):
+        CurvedDetectorGeometry((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (2.0, 1.0), (3, 5))
+    cone = DetectorGeometry((-50.0, 0.0, 0.0), (30.0, -1.0, -0.5), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (0.5, 0.5),
+                            (3, 4))
+    batch = BatchedGeometry((cone, parallel, CurvedDetectorGeometry((0.0, 0.0, 0.0), (100.0, 0.0, -1.0),
+                                                                    (0.0, 1.0, 0.0), (0.0, 0.0, 1.0),
+                                                                    (2.0, 1.0), (3, 4))))
+    assert not batch.flat_cone and batch.shape == (3, 3, 4)
+    value = configuration(GridSpec((4, 5, 6), (1.0, 1.0, 1.0)), batch, ProjectionSpec(16, "float64"))
+    decoded = decode(value)
+    assert decoded.geometry == batch
+    assert [type(v).__name__ for v in decoded.geometry.views] == [
+        "DetectorGeometry", "ParallelGeometry", "CurvedDetectorGeometry"]
