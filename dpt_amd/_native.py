@@ -24,6 +24,8 @@ EXPORTS = (
     "dpt_plan_destroy",
     "dpt_forward",
     "dpt_vjp",
+    "dpt_ray_vjp",
+    "dpt_reduce_rays",
     "dpt_adjoint_gather",
     "dpt_status",
     "dpt_convert_field",
@@ -182,6 +184,10 @@ def lib():
         h.dpt_forward.restype = ctypes.c_int
         h.dpt_vjp.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp]
         h.dpt_vjp.restype = ctypes.c_int
+        h.dpt_ray_vjp.argtypes = [vp, vp, vp, vp, vp, u32, vp]
+        h.dpt_ray_vjp.restype = ctypes.c_int
+        h.dpt_reduce_rays.argtypes = [vp, vp, vp, i32, vp, u32, vp]
+        h.dpt_reduce_rays.restype = ctypes.c_int
         h.dpt_adjoint_gather.argtypes = [vp, vp, vp, vp, i32, u32, vp]
         h.dpt_adjoint_gather.restype = ctypes.c_int
         h.dpt_status.argtypes = [vp, vp, i32, vp]
@@ -305,6 +311,12 @@ class Plan:
             lib().dpt_vjp(self.handle, field, poses, cotangent, grad_field, grad_pose, flags, stream),
             "dpt_vjp",
         )
+
+    def ray_vjp(self, field: int, poses: int, cotangent: int, out: int, flags: int, stream: int) -> None:
+        check(lib().dpt_ray_vjp(self.handle, field, poses, cotangent, out, flags, stream), "dpt_ray_vjp")
+
+    def reduce_rays(self, rays: int, jacobians: int, components: int, out: int, flags: int, stream: int) -> None:
+        check(lib().dpt_reduce_rays(self.handle, rays, jacobians, components, out, flags, stream), "dpt_reduce_rays")
 
     def adjoint_gather(self, poses: int, cotangent: int, out: int, out_precision: str, flags: int,
                        stream: int) -> None:

@@ -20,6 +20,9 @@ constexpr int kBlock = 256;
 constexpr int kTile = 16;
 constexpr int kXcds = 8;
 constexpr int kMaxComponents = 12;
+// Component count that selects per-ray source/endpoint derivatives (6 values
+// per ray, stored per ray instead of reduced) in the pose epilogue.
+constexpr int kRayCoordinates = 7;
 
 struct Params {
     int nx, ny, nz;
@@ -656,7 +659,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int C>
 __device__ __forceinline__ void pose_epilogue(const Params &P, const double *pose, const World &w,
                                               double factor, const double *og, const double *dg,
-                                              double *out) {
+                                              double *out, double seed = 0.0, double integral = 0.0,
+                                              bool fp64_form = false) {
     double sa[3], da[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -665,7 +669,24 @@ __device__ __forceinline__ void pose_epilogue(const Params &P, const double *pos
         sa[i] = factor * s;
         da[i] = factor * t;
     }
-    if constexpr (C == 12) {
+    if constexpr (C == kRayCoordinates) {
+        // kernels/projection_fp32.py:570-583 / kernels/projection.py:1187-1200:
+        // L = |e - s| I(s, e - s), so the segment norm adds seed * I * (e - s)/|e - s|
+        // to the endpoint; the FP64 path forms that product range-safely.
+        double sw[3], dw[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            sw[i] = pose[3 * i] * sa[0] + pose[3 * i + 1] * sa[1] + pose[3 * i + 2] * sa[2];
+            dw[i] = pose[3 * i] * da[0] + pose[3 * i + 1] * da[1] + pose[3 * i + 2] * da[2];
+        }
+        const double length = sqrt(w.wd[0] * w.wd[0] + w.wd[1] * w.wd[1] + w.wd[2] * w.wd[2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            dw[a] += fp64_form ? product3(seed, integral, w.wd[a] / length) : seed * integral * w.wd[a] / length;
+            out[a] = sw[a] - dw[a];
+            out[a + 3] = dw[a];
+        }
+    } else if constexpr (C == 12) {
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll

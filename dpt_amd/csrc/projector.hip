@@ -129,6 +129,26 @@ __device__ __forceinline__ void wave_store(const double *g, double *partial, boo
     }
 }
 
+// Store the epilogue's result: per ray for ray coordinates (6 values at
+// partials + 6 * ray, non-finite flagged), else the wave tile's sum.
+template <int C>
+__device__ __forceinline__ void store_gradient(const Params &P, const double *g, double *partials, int wslot,
+                                               bool wave_valid, int view, int row, int col) {
+    if constexpr (C == kRayCoordinates) {
+        if (!ray_in_range(P, view, row, col)) return;
+        const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+        bool bad = false;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            partials[6 * ray + c] = g[c];
+            bad |= !isfinite(g[c]);
+        }
+        if (bad) flag(P, view, DPT_STATUS_RANGE);
+    } else {
+        wave_store<C>(g, partials + (size_t)wslot * C, wave_valid);
+    }
+}
+
 // Register cell accumulator for the volume cotangent (reference Vec8 sums,
 // kernels/projection_fp32.py:261-269): increments of consecutive samples in one
 // interpolation cell are summed in FP32 registers, then flushed once.
@@ -590,9 +610,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             const float factor = (seed * len) / (float)P.samples;
             World w;  // recomputed: keeps 24 FP64 registers out of the sample loop
             ray_world(P, pose, view, row, col, w);
-            pose_epilogue<C>(P, pose, w, (double)factor, og, dg, gradient);
+            pose_epilogue<C>(P, pose, w, (double)factor, og, dg, gradient, (double)seed, (double)(m * step));
         }
-        wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+        store_gradient<C>(P, gradient, partials, wslot, wave_valid, view, row, col);
     }
 }
 
@@ -877,7 +897,7 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
             ray_world(P, pose, view, row, col, w);
             const double f = factor / S;
             if (isfinite(f)) {
-                pose_epilogue<C>(P, pose, w, f, og, dg, gradient);
+                pose_epilogue<C>(P, pose, w, f, og, dg, gradient, seed, mean * step, true);
             } else {
                 // kernels/projection.py:1163-1183 range-safe products
                 double sa[3], da[3];
@@ -890,10 +910,10 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                 }
                 Params unit = P;
                 for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
-                pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
+                pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient, seed, mean * step, true);
             }
         }
-        wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+        store_gradient<C>(P, gradient, partials, wslot, wave_valid, view, row, col);
     }
 }
 
@@ -1125,6 +1145,7 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
             if (span > 0.0) {
                 const double factor = seed * r.len;
                 double og[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
+                double integral = 0.0;  // sum of radius x node value (ray coordinates)
                 CellWalk walk;
                 begin_cells(P, r, iv, walk);
                 double start = iv.lower;
@@ -1141,8 +1162,12 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                     if constexpr (POSE) {
                         double g1[3], g2[3];
                         const int *lw = s.shared ? s.lower : nullptr;
-                        node_value<T, true>(P, field, s.first, lw, g1);
-                        node_value<T, true>(P, field, s.second, lw, g2);
+                        const double v1 = node_value<T, true>(P, field, s.first, lw, g1);
+                        const double v2 = node_value<T, true>(P, field, s.second, lw, g2);
+                        if constexpr (C == kRayCoordinates) {
+                            integral += s.radius * v1;
+                            integral += s.radius * v2;
+                        }
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
                             og[a] += s.radius * g1[a];
@@ -1223,7 +1248,7 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                         dg[a] += upper_value * up_d - lower_value * lo_d;
                     }
                     if (isfinite(factor)) {
-                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient);
+                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient, seed, integral, true);
                     } else {
                         double sa[3], da[3];
 #pragma unroll
@@ -1235,13 +1260,13 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                         }
                         Params unit = P;
                         for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
-                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient);
+                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient, seed, integral, true);
                     }
                 }
             }
         }
     }
-    if constexpr (POSE) wave_store<C>(gradient, partials + (size_t)wslot * C, wave_valid);
+    if constexpr (POSE) store_gradient<C>(P, gradient, partials, wslot, wave_valid, view, row, col);
 }
 
 // ------------------------------------------------------- small kernels ----
@@ -1460,6 +1485,7 @@ struct dpt_plan {
     int vjp_vb = 1;          // views per workgroup of the VJP launches
     int forward_vb = 1;      // views per workgroup of the forward launch
     bool timing = false;
+    double *ray_out = nullptr;             // destination of a ray-coordinate VJP (caller's buffer)
     GatherRay *d_gather_rays = nullptr;    // adjoint gather workspace, allocated at its first call
     GatherView *d_gather_views = nullptr;
     int *d_guard = nullptr;
@@ -1597,20 +1623,27 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, b
     return DPT_OK;
 }
 
+// Pose partials, or the caller's per-ray buffer for ray coordinates
+template <int C>
+double *pose_destination(dpt_plan *p) {
+    return C == kRayCoordinates ? p->ray_out : p->d_partials;
+}
+
 template <typename T, int C, int LAYOUT>
 void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, hipStream_t s) {
     constexpr bool HALF = !std::is_same<T, float>::value;
     const dim3 grid(tile_grid(P)), block(kBlock);
     const size_t lds = (size_t)P.brick_cap * sizeof(double);
+    double *partials = pose_destination<C>(p);
     if (vol && pose)
         hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd, p->d_acc,
-                           p->d_partials);
+                           partials);
     else if (pose)
         hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, LAYOUT>), grid, block, 0, s, P, f, sd, p->d_acc,
-                           p->d_partials);
+                           partials);
     else if constexpr (LAYOUT != kLayoutCells)  // a volume-only VJP gathers nothing
         hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd,
-                           p->d_acc, p->d_partials);
+                           p->d_acc, partials);
 }
 
 template <typename T, int C>
@@ -1629,27 +1662,29 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
             launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
     } else if (p->cell_gauss) {
         const double *sd = static_cast<const double *>(seeds);
+        double *partials = pose_destination<C>(p);
         if (vol && pose)
             hipLaunchKernelGGL((vjp_gauss_kernel<T, true, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
         else if (vol)
             hipLaunchKernelGGL((vjp_gauss_kernel<T, true, false, C>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
         else
             hipLaunchKernelGGL((vjp_gauss_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
     } else {
         const double *sd = static_cast<const double *>(seeds);
         const size_t lds = (size_t)P.brick_cap * sizeof(double);
+        double *partials = pose_destination<C>(p);
         if (vol && pose)
             hipLaunchKernelGGL((vjp64_kernel<T, true, true, C>), grid, block, lds, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
         else if (vol)
             hipLaunchKernelGGL((vjp64_kernel<T, true, false, C>), grid, block, lds, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
         else
             hipLaunchKernelGGL((vjp64_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
-                               p->d_partials);
+                               partials);
     }
     HIP_TRY(hipGetLastError());
     return DPT_OK;
@@ -1658,6 +1693,7 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
 template <typename T>
 int launch_vjp(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
                int components, bool cells, hipStream_t s) {
+    if (components == kRayCoordinates) return launch_vjp_c<T, kRayCoordinates>(p, P, field, seeds, false, true, cells, s);
     if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, cells, s);
     return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, cells, s);
 }
@@ -2038,6 +2074,111 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     if (vol)
         hipLaunchKernelGGL(finish_volume_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc,
                            grad_field, p->voxels, accumulate, p->d_status);
+    HIP_TRY(hipGetLastError());
+    return DPT_OK;
+}
+
+int dpt_ray_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, double *out_rays,
+                uint32_t flags, void *stream) {
+    if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr || out_rays == nullptr)
+        return fail(DPT_E_INVALID, "plan, field, poses, cotangent and out_rays are required");
+    if (flags & ~(uint32_t)DPT_NO_CHECKS) return fail(DPT_E_INVALID, "ray-coordinate VJP flags are DPT_NO_CHECKS");
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Params P = launch_params(p, poses);
+    configure_tiles(P, p->vjp_vb);
+    if (!(flags & DPT_NO_CHECKS)) {
+        switch (p->storage) {
+        case DPT_STORE_F32:
+            hipLaunchKernelGGL(check_field_kernel<float>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                               static_cast<const float *>(field), p->voxels, p->d_status);
+            break;
+        case DPT_STORE_BF16:
+            hipLaunchKernelGGL(check_field_kernel<__hip_bfloat16>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0,
+                               s, static_cast<const __hip_bfloat16 *>(field), p->voxels, p->d_status);
+            break;
+        default:
+            hipLaunchKernelGGL(check_field_kernel<__half>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                               static_cast<const __half *>(field), p->voxels, p->d_status);
+        }
+        hipLaunchKernelGGL(check_poses_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, poses, P.views,
+                           p->d_status);
+        if (p->arithmetic == DPT_ARITH_FP32)
+            hipLaunchKernelGGL(check_finite_kernel<float>, dim3(grid_blocks(p->pixels)), dim3(kBlock), 0, s,
+                               static_cast<const float *>(cotangent), p->pixels, p->d_status);
+        else
+            hipLaunchKernelGGL(check_finite_kernel<double>, dim3(grid_blocks(p->pixels)), dim3(kBlock), 0, s,
+                               static_cast<const double *>(cotangent), p->pixels, p->d_status);
+    }
+    HIP_TRY(hipGetLastError());
+    if (p->cells) {
+        const int rc = refresh_cells(p, field, s);
+        if (rc != DPT_OK) return rc;
+    }
+    p->ray_out = out_rays;
+    int rc;
+    if (p->storage == DPT_STORE_F32)
+        rc = launch_vjp<float>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+    else if (p->storage == DPT_STORE_BF16)
+        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+    else
+        rc = launch_vjp<__half>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+    p->ray_out = nullptr;
+    return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-view sum of ray-coordinate derivatives contracted with the view's
+// acquisition Jacobian: out[v, k] = sum_p sum_j rays[v, p, j] * J[v, k, j]
+// (kernels/batched_projection.py:15-89, fixed-order FP64 tree per view).
+__global__ __launch_bounds__(kBlock) void rays_reduce_kernel(const double *__restrict__ rays,
+                                                             const double *__restrict__ jacobians, int64_t pixels,
+                                                             int components, double *out, int accumulate,
+                                                             int *status) {
+    const int view = blockIdx.x;
+    __shared__ double red[kBlock];
+    const double *base = rays + (size_t)view * pixels * 6;
+    for (int k = 0; k < components; ++k) {
+        const double *jac = jacobians + ((size_t)view * components + k) * 6;
+        double v = 0.0;
+        for (int64_t i = threadIdx.x; i < pixels; i += kBlock) {
+            const double *g = base + 6 * (size_t)i;
+            v += g[0] * jac[0] + g[1] * jac[1] + g[2] * jac[2] + g[3] * jac[3] + g[4] * jac[4] + g[5] * jac[5];
+        }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+            if (threadIdx.x < stride) red[threadIdx.x] += red[threadIdx.x + stride];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double result = red[0];
+            if (accumulate) result += out[(size_t)view * components + k];
+            out[(size_t)view * components + k] = result;
+            if (!isfinite(result)) atomicOr(&status[view], DPT_STATUS_RANGE);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpt_reduce_rays(dpt_plan *p, const double *rays, const double *jacobians, int32_t components, double *out,
+                    uint32_t flags, void *stream) {
+    if (p == nullptr || rays == nullptr || jacobians == nullptr || out == nullptr)
+        return fail(DPT_E_INVALID, "plan, rays, jacobians and out are required");
+    if (components < 1 || components > 16) return fail(DPT_E_INVALID, "components must be 1..16");
+    if (flags & ~(uint32_t)DPT_ACCUMULATE) return fail(DPT_E_INVALID, "reduction flags are DPT_ACCUMULATE");
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t per_view = p->pixels / p->base.views;
+    hipLaunchKernelGGL(rays_reduce_kernel, dim3(p->base.views), dim3(kBlock), 0, s, rays, jacobians, per_view,
+                       components, out, (flags & DPT_ACCUMULATE) ? 1 : 0, p->d_status);
     HIP_TRY(hipGetLastError());
     return DPT_OK;
 }

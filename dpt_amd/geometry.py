@@ -430,6 +430,19 @@ class ConeBeamIntrinsics:
     def batch(self, angles_rad) -> BatchedGeometry:
         return BatchedGeometry(tuple(self.view(float(a)) for a in angles_rad))
 
+    def ray_derivatives(self, angle_rad: float) -> tuple[tuple[Vector3, Vector3], ...]:
+        """(d source, d endpoint) of every pixel of the view at ``angle_rad``
+        for SOD, SDD, column offset and row offset, in world mm per mm
+        (geometry.py:688-701): the source sits at -SOD along the rotated y
+        axis, the detector plane at SDD - SOD, the offsets move the panel
+        along the rotated x and world z axes."""
+        angle = finite_tuple((float(angle_rad),), "view angle", minimum=None)[0]
+        x = (math.cos(angle), math.sin(angle), 0.0)
+        y = (-math.sin(angle), math.cos(angle), 0.0)
+        minus_y = (-y[0], -y[1], -y[2])
+        zero = (0.0, 0.0, 0.0)
+        return ((minus_y, minus_y), (zero, y), (zero, x), (zero, (0.0, 0.0, 1.0)))
+
 
 def _skew(w) -> Matrix3:
     x, y, z = w

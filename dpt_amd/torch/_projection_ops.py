@@ -172,6 +172,48 @@ def _backward(ctx, cotangent):
 project.register_autograd(_backward, setup_context=_setup_context)
 
 
+def ray_vjp(field: torch.Tensor, poses: torch.Tensor, cotangent: torch.Tensor, value: str) -> torch.Tensor:
+    """Per ray: cotangent x d(optical depth)/d(world source, world endpoint),
+    shape (views, rows, columns, 6) float64 (projection_ray_vjp,
+    python/dpt/batched_projection.py:37-111).  Field, pose and grid are held
+    fixed; the plan's arithmetic and integration rule are used."""
+    shape, dtype = _metadata(field, poses, value)
+    if tuple(cotangent.shape) != shape or cotangent.dtype != dtype or cotangent.device != field.device:
+        raise ContractError("cotangent must match the projected image shape, dtype and device")
+    _require_device(field)
+    stream = torch.cuda.current_stream(field.device).cuda_stream
+    entry = prepare(value, field.device.index or 0, stream, 0)
+    with entry.lock, torch.no_grad():
+        packed = poses.contiguous()
+        seed = cotangent.contiguous()
+        out = torch.empty((*shape, 6), dtype=torch.float64, device=field.device)
+        pointer = _field_pointer(entry, field, stream)
+        entry.plan.ray_vjp(pointer, packed.data_ptr(), seed.data_ptr(), out.data_ptr(), 0, stream)
+        consume(entry, stream)
+    return out
+
+
+def reduce_rays(rays: torch.Tensor, jacobians: torch.Tensor, value: str) -> torch.Tensor:
+    """out[v, k] = sum_p rays[v, p, :] . jacobians[v, k, :] in a fixed order
+    (the intrinsic reduction of batched_projection_vjp,
+    kernels/batched_projection.py:15-89)."""
+    decoded = decode(value)
+    views = decoded.views
+    if tuple(rays.shape) != (*decoded.image_shape, 6) or rays.dtype != torch.float64:
+        raise ContractError("rays must be float64 (views, rows, columns, 6)")
+    if jacobians.dim() != 3 or jacobians.shape[0] != views or jacobians.shape[2] != 6:
+        raise ContractError("jacobians must be float64 (views, components, 6)")
+    _require_device(rays)
+    stream = torch.cuda.current_stream(rays.device).cuda_stream
+    entry = prepare(value, rays.device.index or 0, stream, 0)
+    jac = jacobians.to(device=rays.device, dtype=torch.float64).contiguous()
+    out = torch.empty((views, jac.shape[1]), dtype=torch.float64, device=rays.device)
+    with entry.lock:
+        entry.plan.reduce_rays(rays.contiguous().data_ptr(), jac.data_ptr(), jac.shape[1], out.data_ptr(), 0, stream)
+        consume(entry, stream)
+    return out
+
+
 def adjoint_gather(poses: torch.Tensor, cotangent: torch.Tensor, value: str, out: torch.Tensor,
                    accumulate: bool) -> torch.Tensor:
     """Deterministic FP64 transpose: out[v] (+)= sum_p A(poses)[p, v] cotangent[p].

@@ -25,7 +25,7 @@ from collections.abc import Sequence
 import torch
 
 from ..contracts import ContractError, finite_scalar, integer
-from ..geometry import PANEL_TYPES, BatchedGeometry, RayBundleGeometry, RigidTransform
+from ..geometry import PANEL_TYPES, BatchedGeometry, ConeBeamIntrinsics, RayBundleGeometry, RigidTransform
 from ..projection import ProjectionSpec
 from ..volumes import GridSpec
 from . import _projection_ops
@@ -189,6 +189,38 @@ class Projector(torch.nn.Module):
             field.requires_grad and torch.is_grad_enabled(),
             poses.requires_grad and torch.is_grad_enabled(),
         )
+
+    def ray_vjp(self, field: torch.Tensor, cotangent: torch.Tensor, pose: torch.Tensor | None = None, *,
+                packed_pose=None) -> torch.Tensor:
+        """Per ray: cotangent x d(optical depth)/d(world source xyz, world
+        endpoint xyz), shape (views, rows, columns, 6) float64, with the field
+        and object pose fixed (projection_ray_vjp,
+        python/dpt/batched_projection.py:37-111)."""
+        if not isinstance(field, torch.Tensor) or tuple(field.shape) != self.grid.shape:
+            raise ContractError("field must be a Torch tensor with grid shape (nz,ny,nx)")
+        poses = self._poses(field, pose, packed_pose).detach()
+        return _projection_ops.ray_vjp(field.detach(), poses.contiguous(), cotangent, self._configuration)
+
+    def intrinsics_vjp(self, field: torch.Tensor, cotangent: torch.Tensor, intrinsics, angles_rad,
+                       pose: torch.Tensor | None = None, *, packed_pose=None) -> torch.Tensor:
+        """Cone intrinsic cotangent (views, 4) = SOD, SDD, column offset, row
+        offset per view (batched_projection_vjp with active intrinsics,
+        python/dpt/batched_projection.py:340-539); sum the rows for a shared
+        calibration.  The module's geometry must be ``intrinsics`` viewed at
+        ``angles_rad`` (one record, or one per angle)."""
+        angles = [float(a) for a in angles_rad]
+        records = list(intrinsics) if isinstance(intrinsics, (list, tuple)) else [intrinsics] * len(angles)
+        if len(records) != len(angles) or any(not isinstance(r, ConeBeamIntrinsics) for r in records):
+            raise ContractError("intrinsics must be one cone record or one per angle")
+        expected = BatchedGeometry(tuple(r.view(a) for r, a in zip(records, angles)))
+        if expected != self.geometry:
+            raise ContractError("the module's geometry is not these intrinsics at these angles")
+        rays = self.ray_vjp(field, cotangent, pose, packed_pose=packed_pose)
+        jac = torch.tensor(
+            [[[*ds, *de] for ds, de in r.ray_derivatives(a)] for r, a in zip(records, angles)],
+            dtype=torch.float64, device=rays.device,
+        )
+        return _projection_ops.reduce_rays(rays, jac, self._configuration)
 
     def adjoint_gather(
         self,

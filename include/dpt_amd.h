@@ -160,6 +160,31 @@ int dpt_vjp(dpt_plan *plan, const void *field, const double *poses, const void *
             float *grad_field, double *grad_pose, uint32_t flags, void *stream);
 
 /*
+ * Ray-coordinate VJP: out_rays (views, rows, cols, 6) float64 = cotangent[p]
+ * x d(optical depth)/d(world source xyz, world endpoint xyz) of each ray,
+ * stored per ray (no reduction), through the plan's arithmetic and
+ * integration rule (clipping, sample motion and the segment norm included).
+ * cotangent as for dpt_vjp; flags: DPT_NO_CHECKS.  Replaces
+ * projection_ray_vjp (python/dpt/batched_projection.py:37-111) over the
+ * ray_coordinates branch of kernels/projection_fp32.py:570-583 and
+ * kernels/projection.py:1184-1200.
+ */
+int dpt_ray_vjp(dpt_plan *plan, const void *field, const double *poses, const void *cotangent,
+                double *out_rays, uint32_t flags, void *stream);
+
+/*
+ * out (views, components) float64 = per view, sum over its pixels of
+ * rays[v, p, :] . jacobians[v, k, :] (rays from dpt_ray_vjp; jacobians
+ * device float64 (views, components, 6): d(source, endpoint)/d(parameter k)
+ * in world mm), in a fixed order; DPT_ACCUMULATE adds to out.  With the cone
+ * intrinsic Jacobians (SOD, SDD, column offset, row offset) this is the
+ * intrinsics VJP of batched_projection_vjp (python/dpt/batched_projection.py:
+ * 178-209, 433-539; kernels/batched_projection.py:15-89).
+ */
+int dpt_reduce_rays(dpt_plan *plan, const double *rays, const double *jacobians, int32_t components,
+                    double *out, uint32_t flags, void *stream);
+
+/*
  * Deterministic adjoint gather: out[v] = sum over views, pixels (row-major),
  * midpoint samples and cell corners, in that order, of A(poses)[p, v] *
  * cotangent[p], summed in FP64 by one thread per voxel, so the result is the

@@ -363,6 +363,37 @@ static void pose_epilogue(const ref_plan *p, const double *pose, const ref_ray *
     }
 }
 
+/* Ray-coordinate mode (components == REF_RAY_COORDINATES): per ray
+ * cotangent x dL/d(world source, world endpoint), 6 values
+ * (kernels/projection_fp32.py:570-583, kernels/projection.py:1184-1200):
+ * L = |e - s| I(s, e - s), so the endpoint gains seed * I * (e - s)/|e - s|;
+ * the FP64 path forms that product range-safely. */
+#define REF_RAY_COORDINATES 7
+
+static void ray_epilogue(const ref_plan *p, const double *pose, const ref_ray *pr, double factor,
+                         const double *og, const double *dg, double seed, double integral, int fp64_form,
+                         double *out) {
+    double sa[3], da[3], sw[3], dw[3];
+    for (int i = 0; i < 3; ++i) {
+        const double s = p->g[i] * og[0] + p->g[3 + i] * og[1] + p->g[6 + i] * og[2];
+        const double t = p->g[i] * dg[0] + p->g[3 + i] * dg[1] + p->g[6 + i] * dg[2];
+        sa[i] = factor * s;
+        da[i] = factor * t;
+    }
+    for (int i = 0; i < 3; ++i) {
+        sw[i] = pose[3 * i] * sa[0] + pose[3 * i + 1] * sa[1] + pose[3 * i + 2] * sa[2];
+        dw[i] = pose[3 * i] * da[0] + pose[3 * i + 1] * da[1] + pose[3 * i + 2] * da[2];
+    }
+    for (int a = 0; a < 3; ++a) {
+        dw[a] += fp64_form ? product3(seed, integral, pr->world_direction[a] / pr->length)
+                           : seed * integral * pr->world_direction[a] / pr->length;
+        out[a] = sw[a] - dw[a];
+        out[a + 3] = dw[a];
+    }
+}
+
+static int outputs_of(int components) { return components == REF_RAY_COORDINATES ? 6 : components; }
+
 static void neumaier(double *sum, double *comp, double x) {
     double t = *sum + x;
     if (fabs(*sum) >= fabs(x))
@@ -399,7 +430,7 @@ void ref_vjp_f32(const ref_plan *p, const float *field, const double *poses, con
     const int active_pose = grad_pose != NULL || per_ray_out != NULL;
     double *per_ray = per_ray_out;
     if (active_pose && per_ray == NULL)
-        per_ray = (double *)calloc((size_t)total_rays * components, sizeof(double));
+        per_ray = (double *)calloc((size_t)total_rays * outputs_of(components), sizeof(double));
 #pragma omp parallel for schedule(dynamic, 64)
     for (long ray = 0; ray < total_rays; ++ray) {
         int view = (int)(ray / pixels), pixel = (int)(ray % pixels);
@@ -460,12 +491,17 @@ void ref_vjp_f32(const ref_plan *p, const float *field, const double *poses, con
                                              fmaf(lower_seed, iv.lo_d[a], span * ddir[a]));
                     }
                     float factor = (seed * len) / (float)p->samples;
-                    pose_epilogue(p, pose, &pr, (double)factor, og, dg, components, gradient);
+                    if (components == REF_RAY_COORDINATES)
+                        ray_epilogue(p, pose, &pr, (double)factor, og, dg, (double)seed, (double)(mean * step), 0,
+                                     gradient);
+                    else
+                        pose_epilogue(p, pose, &pr, (double)factor, og, dg, components, gradient);
                 }
             }
         }
         if (active_pose)
-            for (int c = 0; c < components; ++c) per_ray[(size_t)ray * components + c] = gradient[c];
+            for (int c = 0; c < outputs_of(components); ++c)
+                per_ray[(size_t)ray * outputs_of(components) + c] = gradient[c];
     }
     if (grad_pose != NULL) reduce_pose(p, per_ray, components, grad_pose);
     if (per_ray != per_ray_out) free(per_ray);
@@ -614,7 +650,7 @@ void ref_vjp_f64(const ref_plan *p, const float *field, const double *poses, con
     const int active_pose = grad_pose != NULL || per_ray_out != NULL;
     double *per_ray = per_ray_out;
     if (active_pose && per_ray == NULL)
-        per_ray = (double *)calloc((size_t)total_rays * components, sizeof(double));
+        per_ray = (double *)calloc((size_t)total_rays * outputs_of(components), sizeof(double));
     const int pow2 = p->samples > 0 && (p->samples & (p->samples - 1)) == 0;
 #pragma omp parallel for schedule(dynamic, 64)
     for (long ray = 0; ray < total_rays; ++ray) {
@@ -685,7 +721,9 @@ void ref_vjp_f64(const ref_plan *p, const float *field, const double *poses, con
                         dg[a] = span * ddir[a] + lower_seed * iv.lo_d[a] + upper_seed * iv.up_d[a];
                     }
                     factor /= (double)p->samples;
-                    if (isfinite(factor)) {
+                    if (isfinite(factor) && components == REF_RAY_COORDINATES) {
+                        ray_epilogue(p, pose, &r, factor, og, dg, seed, mean * step, 1, gradient);
+                    } else if (isfinite(factor)) {
                         pose_epilogue(p, pose, &r, factor, og, dg, components, gradient);
                     } else {
                         /* kernels/projection.py:1163-1183: range-safe products */
@@ -702,13 +740,17 @@ void ref_vjp_f64(const ref_plan *p, const float *field, const double *poses, con
                         ref_plan unit = *p;
                         memset(unit.g, 0, sizeof unit.g);
                         unit.g[0] = unit.g[4] = unit.g[8] = 1.0;
-                        pose_epilogue(&unit, pose, &r, 1.0, sa_unit, da_unit, components, gradient);
+                        if (components == REF_RAY_COORDINATES)
+                            ray_epilogue(&unit, pose, &r, 1.0, sa_unit, da_unit, seed, mean * step, 1, gradient);
+                        else
+                            pose_epilogue(&unit, pose, &r, 1.0, sa_unit, da_unit, components, gradient);
                     }
                 }
             }
         }
         if (active_pose)
-            for (int c = 0; c < components; ++c) per_ray[(size_t)ray * components + c] = gradient[c];
+            for (int c = 0; c < outputs_of(components); ++c)
+                per_ray[(size_t)ray * outputs_of(components) + c] = gradient[c];
     }
     if (grad_pose != NULL) reduce_pose(p, per_ray, components, grad_pose);
     if (per_ray != per_ray_out) free(per_ray);
@@ -998,6 +1040,7 @@ typedef struct {
     double *grad_field;
     int pose;
     double og[3], dg[3];
+    double integral; /* sum of radius x node value */
 } vjp_ctx;
 
 /* kernels/projection.py:375-393 / 405-430 with FP32 increments, accumulated
@@ -1022,8 +1065,10 @@ static void vjp_piece(const cell_piece *s, void *vctx) {
     if (c->pose) {
         double g1[3], g2[3];
         const int *lower = s->shared ? s->lower : NULL;
-        sample_point64(c->field, s->first, c->shape, lower, g1);
-        sample_point64(c->field, s->second, c->shape, lower, g2);
+        const double v1 = sample_point64(c->field, s->first, c->shape, lower, g1);
+        const double v2 = sample_point64(c->field, s->second, c->shape, lower, g2);
+        c->integral += s->radius * v1;
+        c->integral += s->radius * v2;
         for (int a = 0; a < 3; ++a) {
             c->og[a] += s->radius * g1[a];
             c->og[a] += s->radius * g2[a];
@@ -1071,7 +1116,7 @@ void ref_vjp_gauss_f64(const ref_plan *p, const float *field, const double *pose
     const int active_pose = grad_pose != NULL || per_ray_out != NULL;
     double *per_ray = per_ray_out;
     if (active_pose && per_ray == NULL)
-        per_ray = (double *)calloc((size_t)total_rays * components, sizeof(double));
+        per_ray = (double *)calloc((size_t)total_rays * outputs_of(components), sizeof(double));
 #pragma omp parallel for schedule(dynamic, 64)
     for (long ray = 0; ray < total_rays; ++ray) {
         const int view = (int)(ray / pixels), pixel = (int)(ray % pixels);
@@ -1114,7 +1159,9 @@ void ref_vjp_gauss_f64(const ref_plan *p, const float *field, const double *pose
                         c.og[a] += upper_value * iv.up_o[a] - lower_value * iv.lo_o[a];
                         c.dg[a] += upper_value * iv.up_d[a] - lower_value * iv.lo_d[a];
                     }
-                    if (isfinite(c.factor)) {
+                    if (isfinite(c.factor) && components == REF_RAY_COORDINATES) {
+                        ray_epilogue(p, pose, &r, c.factor, c.og, c.dg, seed, c.integral, 1, gradient);
+                    } else if (isfinite(c.factor)) {
                         pose_epilogue(p, pose, &r, c.factor, c.og, c.dg, components, gradient);
                     } else {
                         double sa[3], da[3];
@@ -1127,13 +1174,17 @@ void ref_vjp_gauss_f64(const ref_plan *p, const float *field, const double *pose
                         ref_plan unit = *p;
                         memset(unit.g, 0, sizeof unit.g);
                         unit.g[0] = unit.g[4] = unit.g[8] = 1.0;
-                        pose_epilogue(&unit, pose, &r, 1.0, sa, da, components, gradient);
+                        if (components == REF_RAY_COORDINATES)
+                            ray_epilogue(&unit, pose, &r, 1.0, sa, da, seed, c.integral, 1, gradient);
+                        else
+                            pose_epilogue(&unit, pose, &r, 1.0, sa, da, components, gradient);
                     }
                 }
             }
         }
         if (active_pose)
-            for (int k = 0; k < components; ++k) per_ray[(size_t)ray * components + k] = gradient[k];
+            for (int k = 0; k < outputs_of(components); ++k)
+                per_ray[(size_t)ray * outputs_of(components) + k] = gradient[k];
     }
     if (grad_pose != NULL) reduce_pose(p, per_ray, components, grad_pose);
     if (per_ray != per_ray_out) free(per_ray);

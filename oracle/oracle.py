@@ -41,6 +41,7 @@ _LIB_PATH = _HERE / "_build" / "libcpu_ref.so"
 _LIB = None
 
 IDENTITY_POSE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+RAY_COORDINATES = 7  # the C oracle's per-ray source/endpoint derivative mode
 
 
 def build() -> Path:
@@ -257,6 +258,23 @@ class Scene:
         fn(ctypes.byref(self.plan), field.ctypes.data, poses.ctypes.data, seeds.ctypes.data,
            ptr(grad_field), ptr(grad_pose), ptr(rays), components, status.ctypes.data)
         return grad_field, grad_pose, rays, status
+
+    def ray_vjp(self, field, cotangent, poses=None, *, precision: str = "fp32", integration: str = "midpoint"):
+        """Per ray: cotangent x d(optical depth)/d(world source xyz, world
+        endpoint xyz) (kernels/projection_fp32.py:570-583,
+        kernels/projection.py:1184-1200).  Returns ((V*R*C, 6), status)."""
+        field = np.ascontiguousarray(field, dtype=np.float32).reshape(-1)
+        poses = self._poses(poses)
+        seed_dtype = np.float32 if precision == "fp32" else np.float64
+        seeds = np.ascontiguousarray(cotangent, dtype=seed_dtype).reshape(-1)
+        rays = np.zeros((seeds.size, 6), dtype=np.float64)
+        status = np.zeros(self.views, dtype=np.int32)
+        fn = _lib().ref_vjp_f32 if precision == "fp32" else _lib().ref_vjp_f64
+        if integration == "cell_gauss":
+            fn = _lib().ref_vjp_gauss_f64
+        fn(ctypes.byref(self.plan), field.ctypes.data, poses.ctypes.data, seeds.ctypes.data, None, None,
+           rays.ctypes.data, RAY_COORDINATES, status.ctypes.data)
+        return rays, status
 
     def gather(self, cotangent, poses=None, *, previous=None, precision: str = "float64"):
         """Deterministic FP64 adjoint gather (kernels/projection_adjoint.py:73-170).

@@ -424,3 +424,36 @@ def test_cell_gauss_vjp_is_the_adjoint_and_the_pose_derivative(name):
         _, packed, _, _ = scene.vjp(values, seeds, pose, precision="fp64", volume=False, integration="cell_gauss")
         _, midpoint_packed, _, _ = scene.vjp(values, seeds, pose, precision="fp64", volume=False)
         assert np.isfinite(packed).all() and not np.array_equal(packed, midpoint_packed)
+
+
+# ---------------------------------------------------------------------------
+# Ray-coordinate derivatives (python/dpt/batched_projection.py:37-111)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("mode", [("fp64", "midpoint"), ("fp64", "cell_gauss"), ("fp32", "midpoint")])
+def test_ray_coordinate_vjp_matches_central_differences(mode):
+    """Each ray's 6 derivatives against central differences of its own
+    optical depth when its world source or endpoint moves along one axis
+    (the oblique bundle: axis-aligned, oblique and missing rays)."""
+    precision, integration = mode
+    grid, sources, endpoints, pose, values = scenes.oblique_scene()
+    samples = 97
+    scene = O.Scene(grid, samples, rays=(sources, endpoints))
+    seeds = np.asarray([0.7, -0.3, 0.2, 0.6, 0.0])
+    rays, status = scene.ray_vjp(values, seeds, pose, precision=precision, integration=integration)
+    assert status.tolist() == [0]
+    assert np.abs(rays[4]).max() == 0.0  # zero cotangent
+    h = 1e-4 if precision == "fp64" else 2e-2
+    tol = 2e-5 if precision == "fp64" else 5e-3
+    for ray in range(4):
+        for j in range(6):
+            def depth(delta):
+                s, e = sources.copy(), endpoints.copy()
+                (s if j < 3 else e)[ray, j % 3] += delta
+                image, _ = O.Scene(grid, samples, rays=(s, e)).forward(
+                    values, pose, precision=precision, integration=integration)
+                return float(image.reshape(-1)[ray])
+            fd = seeds[ray] * (depth(h) - depth(-h)) / (2 * h)
+            scale = max(1e-3, np.abs(rays[ray]).max())
+            assert abs(rays[ray, j] - fd) <= tol * scale, (ray, j, rays[ray, j], fd)
