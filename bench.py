@@ -56,6 +56,8 @@ def _args():
     parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
     parser.add_argument("--no-gather", action="store_true", help="skip the deterministic adjoint gather measurement")
     parser.add_argument("--no-cell-gauss", action="store_true", help="skip the exact cell-Gauss measurement")
+    parser.add_argument("--cell-gauss-only", action="store_true",
+                        help="only the exact cell-Gauss measurement (profiling driver)")
     parser.add_argument("--gather-only", action="store_true",
                         help="only the deterministic adjoint gather measurement (profiling driver)")
     parser.add_argument("--transport-only", action="store_true",
@@ -649,6 +651,9 @@ def main() -> int:
     if args.gather_only:
         print(json.dumps(gather_benchmark(grid, geometry, values, target, args.samples)), flush=True)
         return 0
+    if args.cell_gauss_only:
+        print(json.dumps(cell_gauss_benchmark(grid, geometry, values, target, args.samples)), flush=True)
+        return 0
 
     pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
     projector = make()
@@ -716,9 +721,12 @@ def main() -> int:
         image = module(frozen_field, pose)
         torch.autograd.grad(image, pose, seed_image)
 
+    def ray_vjp():
+        module.ray_vjp(frozen_field, seed_image, frozen_pose)
+
     components = {}
     for name, fn, ref_ms in (("forward", forward_only, 512.60), ("forward_volume_vjp", volume_vjp, 1058.0),
-                             ("forward_pose_vjp", pose_vjp, None)):
+                             ("forward_pose_vjp", pose_vjp, None), ("ray_coordinate_vjp", ray_vjp, None)):
         t = timed(fn)
         components[name] = {"ms": round(t * 1e3, 3), "gsamples_per_s": round(nominal / t / 1e9, 3)}
         if ref_ms is not None:
