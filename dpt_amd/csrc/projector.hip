@@ -1115,6 +1115,70 @@ __device__ __forceinline__ bool add_increment(double *acc, int index, double con
     return isfinite(rounded);
 }
 
+// Register sums of one cell's 8 corners for the cell-Gauss volume adjoint:
+// consecutive pieces of a ray lie in face-adjacent cells sharing 4 corners,
+// which stay in registers while the other 4 are flushed with FP64 global
+// atomics (half the atomics of one per corner and piece).  Only for grids
+// whose axes all have >= 2 samples (no clamped duplicate corners).
+struct CellCarry {
+    int l[3];  // carried cell; l[0] < 0 when empty
+    double v[8];
+
+    __device__ __forceinline__ void reset() {
+        l[0] = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = 0.0;
+    }
+
+    __device__ __forceinline__ void flush(const Params &P, double *acc) {
+        if (l[0] >= 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (v[k] != 0.0) unsafeAtomicAdd(acc + corner_of(P, l, k), v[k]);
+        }
+        reset();
+    }
+
+    // the next cell is l + dir * e_A: corners on the far face survive
+    template <int A>
+    __device__ __forceinline__ void shift(const Params &P, double *acc, int dir) {
+        constexpr int bit = 1 << A;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k & bit) continue;
+            const int keep = dir > 0 ? (k | bit) : k, drop = dir > 0 ? k : (k | bit);
+            if (v[drop] != 0.0) unsafeAtomicAdd(acc + corner_of(P, l, drop), v[drop]);
+            const double kept = v[keep];
+            v[k] = dir > 0 ? kept : 0.0;
+            v[k | bit] = dir > 0 ? 0.0 : kept;
+        }
+        l[A] += dir;
+    }
+
+    __device__ __forceinline__ void move_to(const Params &P, double *acc, const int *cell) {
+        if (l[0] < 0) {
+            l[0] = cell[0];
+            l[1] = cell[1];
+            l[2] = cell[2];
+            return;
+        }
+        const int dx = cell[0] - l[0], dy = cell[1] - l[1], dz = cell[2] - l[2];
+        if (dx == 0 && dy == 0 && dz == 0) return;
+        if (dy == 0 && dz == 0 && (dx == 1 || dx == -1)) {
+            shift<0>(P, acc, dx);
+        } else if (dx == 0 && dz == 0 && (dy == 1 || dy == -1)) {
+            shift<1>(P, acc, dy);
+        } else if (dx == 0 && dy == 0 && (dz == 1 || dz == -1)) {
+            shift<2>(P, acc, dz);
+        } else {
+            flush(P, acc);
+            l[0] = cell[0];
+            l[1] = cell[1];
+            l[2] = cell[2];
+        }
+    }
+};
+
 // kernels/projection.py:873-1237 with cell_gauss (FP32 volume destination:
 // each increment rounded to FP32, summed here in FP64)
 template <typename T, bool VOL, bool POSE, int C>
@@ -1146,6 +1210,9 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                 const double factor = seed * r.len;
                 double og[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
                 double integral = 0.0;  // sum of radius x node value (ray coordinates)
+                const bool carried = P.nx > 1 && P.ny > 1 && P.nz > 1;
+                CellCarry carry;
+                carry.reset();
                 CellWalk walk;
                 begin_cells(P, r, iv, walk);
                 double start = iv.lower;
@@ -1194,6 +1261,7 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                             }
                             const double w1 = inside_support64(P, s.first) ? vf : 0.0;
                             const double w2 = inside_support64(P, s.second) ? vf : 0.0;
+                            if (carried) carry.move_to(P, acc, s.lower);
 #pragma unroll
                             for (int k = 0; k < 8; ++k) {
                                 double first = w1, second = w2;
@@ -1201,7 +1269,13 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                                 for (int a = 0; a < 3; ++a) first *= ((k >> a) & 1) ? f1[a] : 1.0 - f1[a];
 #pragma unroll
                                 for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
-                                bad |= !add_increment(acc, corner_of(P, s.lower, k), first + second);
+                                if (carried) {
+                                    const float rounded = (float)(first + second);
+                                    bad |= !isfinite(rounded);
+                                    carry.v[k] += (double)rounded;
+                                } else {
+                                    bad |= !add_increment(acc, corner_of(P, s.lower, k), first + second);
+                                }
                             }
                         } else {
                             // kernels/projection.py:437-470 (scatter per node)
@@ -1225,6 +1299,7 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                     start = s.end;
                     advance_cells(P, r, walk, s.end, iv.upper);
                 }
+                if constexpr (VOL) carry.flush(P, acc);
                 bad |= start < iv.upper;
                 if (bad) flag(P, view, DPT_STATUS_RANGE);
                 if constexpr (POSE) {
