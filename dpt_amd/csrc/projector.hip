@@ -154,7 +154,8 @@ __device__ __forceinline__ void store_gradient(const Params &P, const double *g,
 // interpolation cell are summed in FP32 registers, then flushed once.
 struct CellSums {
     int x, y, z;   // lower corner; x < 0 when empty
-    float s[8];
+    float s[8];    // FP32 sums of the corner weights of the cell's samples
+    float scale;   // the ray's volume factor: a flushed corner adds scale * s[k]
     __device__ __forceinline__ void reset() {
         x = -1;
 #pragma unroll
@@ -187,13 +188,13 @@ __device__ __forceinline__ void add_corners(const Params &P, const Brick &B, con
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
-            atomicAdd(&brick[base + off], (double)f[k]);
+            atomicAdd(&brick[base + off], (double)cs.scale * (double)f[k]);
         }
     } else {
         const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (f[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)f[k]);
+            if (f[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.scale * (double)f[k]);
     }
 }
 
@@ -441,6 +442,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     shu.init();
     CellSums cs;
     cs.reset();
+    cs.scale = volume_factor;  // the ray's factor (0 for inactive lanes)
     bool bad = false;
 
     if constexpr (VOL) {
@@ -536,21 +538,14 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 }
                 if constexpr (VOL) {
                     if (bounded) {
-                        // The flush-to-zero rule of scatter_increment changes
-                        // an increment only when its magnitude is below
-                        // 2^-126: apply it when some lane of the wave has one.
-                        float inc[8];
+                        // The cell keeps FP32 sums of the corner weights; the
+                        // ray's factor multiplies each sum once, in FP64, when
+                        // the cell is flushed (instead of rounding seed * w
+                        // per sample and corner: 8 multiplies and the
+                        // flush-to-zero test fewer per sample, a difference
+                        // of an FP32 rounding per cell sum).
 #pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) inc[kk] = volume_factor * wt[kk];
-                        const float smallest =
-                            fminf(fminf(fminf(fabsf(inc[0]), fabsf(inc[1])), fminf(fabsf(inc[2]), fabsf(inc[3]))),
-                                  fminf(fminf(fabsf(inc[4]), fabsf(inc[5])), fminf(fabsf(inc[6]), fabsf(inc[7]))));
-                        if (__builtin_amdgcn_ballot_w64(smallest < 0x1p-126f) != 0) {
-#pragma unroll
-                            for (int kk = 0; kk < 8; ++kk) inc[kk] = scatter_increment(volume_factor, wt[kk]);
-                        }
-#pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += inc[kk];
+                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += wt[kk];
                     } else {
 #pragma unroll
                         for (int kk = 0; kk < 8; ++kk) {
