@@ -129,3 +129,23 @@ def test_projector_ray_vjp_matches_the_oracle(mode):
     expected, _ = O.Scene(grid, 64, panels=panels).ray_vjp(
         values, seed.cpu().numpy(), precision="fp32" if precision == "float32" else "fp64", integration=integration)
     assert _rel(rays.cpu().numpy().reshape(-1, 6), expected) <= (1e-5 if precision == "float32" else 1e-10)
+
+
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_ray_vjp_with_half_storage_matches_the_oracle_on_the_rounded_field(storage):
+    """Half storage rounds the field once; the ray derivatives of that field
+    (FP32 sampling, compensated sums) match the oracle's FP32 path on the
+    rounded values."""
+    from dpt_amd.geometry import DetectorGeometry
+
+    grid = O.Grid((10, 11, 12), (1.0, 1.0, 1.0), (-5.5, -5.0, -4.5))
+    panels = scenes.cone_panels(grid, 2, 8, 9, sod=50.0, sdd=90.0, offset=0.2)
+    geometry = [DetectorGeometry(p.source, p.origin, p.u, p.v, p.spacing, p.shape) for p in panels]
+    module = Projector(grid=GridSpec(grid.shape, grid.spacing, grid.origin), geometry=geometry, samples_per_ray=64,
+                       storage=storage)
+    values = torch.tensor(scenes.smooth_field(grid, 2), device="cuda")
+    rounded = values.to(torch.bfloat16 if storage == "bfloat16" else torch.float16).float()
+    seed = torch.linspace(-1.0, 1.0, 2 * 8 * 9, dtype=torch.float32, device="cuda").reshape(2, 8, 9)
+    rays = module.ray_vjp(values, seed)
+    expected, _ = O.Scene(grid, 64, panels=panels).ray_vjp(rounded.cpu().numpy(), seed.cpu().numpy(), precision="fp32")
+    assert _rel(rays.cpu().numpy().reshape(-1, 6), expected) <= 1e-5
