@@ -1045,6 +1045,24 @@ __device__ __forceinline__ double node_value(const Params &P, const T *field, co
     return sample_cell64<GRAD>(c, v, g);
 }
 
+// Value (and slope) of a node in a shared piece whose 8 corner values v of
+// the cell at lower are already loaded (sample_cell, kernels/projection.py:
+// 278-284); zero outside the support.
+template <bool GRAD>
+__device__ __forceinline__ double node_in_cell(const Params &P, const double *pt, const int *lower, const double *v,
+                                               double *g) {
+    if (GRAD) g[0] = g[1] = g[2] = 0.0;
+    Cell64 c;
+    if (!cell64(P, pt, c)) return 0.0;
+    c.lx = lower[0];
+    c.ly = lower[1];
+    c.lz = lower[2];
+    c.fx = fmin(fmax(pt[0], 0.0), (double)(P.nx - 1)) - (double)c.lx;
+    c.fy = fmin(fmax(pt[1], 0.0), (double)(P.ny - 1)) - (double)c.ly;
+    c.fz = fmin(fmax(pt[2], 0.0), (double)(P.nz - 1)) - (double)c.lz;
+    return sample_cell64<GRAD>(c, v, g);
+}
+
 // Grid index of corner k of the cell at lower (kernels/projection.py:217-222)
 __device__ __forceinline__ int corner_of(const Params &P, const int *lower, int k) {
     const int x = min(lower[0] + (k & 1), P.nx - 1);
@@ -1088,9 +1106,15 @@ __global__ __launch_bounds__(kBlock) void forward_gauss_kernel(Params P, const T
                 break;
             }
             bad |= !finite3d(s.first) || !finite3d(s.second);
-            const int *lw = s.shared ? s.lower : nullptr;
-            integral += s.radius * node_value<T, false>(P, field, s.first, lw, nullptr);
-            integral += s.radius * node_value<T, false>(P, field, s.second, lw, nullptr);
+            if (s.shared) {  // one load of the cell for both nodes
+                double v[8];
+                load8d<T>(P, field, (s.lower[2] * P.ny + s.lower[1]) * P.nx + s.lower[0], v);
+                integral += s.radius * node_in_cell<false>(P, s.first, s.lower, v, nullptr);
+                integral += s.radius * node_in_cell<false>(P, s.second, s.lower, v, nullptr);
+            } else {
+                integral += s.radius * node_value<T, false>(P, field, s.first, nullptr, nullptr);
+                integral += s.radius * node_value<T, false>(P, field, s.second, nullptr, nullptr);
+            }
             start = s.end;
             advance_cells(P, r, walk, s.end, iv.upper);
         }
@@ -1222,10 +1246,16 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                     }
                     bad |= !finite3d(s.first) || !finite3d(s.second);
                     if constexpr (POSE) {
-                        double g1[3], g2[3];
-                        const int *lw = s.shared ? s.lower : nullptr;
-                        const double v1 = node_value<T, true>(P, field, s.first, lw, g1);
-                        const double v2 = node_value<T, true>(P, field, s.second, lw, g2);
+                        double g1[3], g2[3], v1, v2;
+                        if (s.shared) {  // one load of the cell for both nodes
+                            double v[8];
+                            load8d<T>(P, field, (s.lower[2] * P.ny + s.lower[1]) * P.nx + s.lower[0], v);
+                            v1 = node_in_cell<true>(P, s.first, s.lower, v, g1);
+                            v2 = node_in_cell<true>(P, s.second, s.lower, v, g2);
+                        } else {
+                            v1 = node_value<T, true>(P, field, s.first, nullptr, g1);
+                            v2 = node_value<T, true>(P, field, s.second, nullptr, g2);
+                        }
                         if constexpr (C == kRayCoordinates) {
                             integral += s.radius * v1;
                             integral += s.radius * v2;
