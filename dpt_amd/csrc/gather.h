@@ -28,14 +28,19 @@ struct alignas(16) GatherView {
     double norm_a;  // |A|
     double ratio;   // (|B| + |C|) / |A|, the scale of the pixel-coordinate rounding
     double margin;  // support-box widening for this view's rays, grid units
-    int ok;         // 0: cull nothing (explicit rays or a degenerate panel)
-    int pad;
+    int ok;          // 0: cull nothing (explicit rays, cylindrical or degenerate panels)
+    int affine;      // 1: parallel panel, pixel coordinates (x - s).b and (x - s).c with no denominator
+    int cols, rows;  // detector columns and rows of the view; ray (row, col) is at row * cols + col
 };
 
 // Enqueue the gather: guard reset, cotangent/pose checks (checks != 0), view
 // and ray preparation, then one thread per voxel.  out is double (out_fp64)
 // or float; the guard word suppresses every write when a check failed.
+// panel_meta / panel_modes: the device panel records of a plan made from
+// panel records (views x 18, views) with image_cols detector columns, or
+// null.
 hipError_t launch_gather(const Params &P, const double *cotangent, GatherRay *rays, GatherView *views, int *guard,
-                         void *out, bool out_fp64, bool accumulate, bool checks, hipStream_t stream);
+                         void *out, bool out_fp64, bool accumulate, bool checks, const double *panel_meta,
+                         const int *panel_modes, int image_cols, hipStream_t stream);
 
 }  // namespace dpt

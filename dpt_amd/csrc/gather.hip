@@ -22,6 +22,8 @@
 // the sample points, so every skipped sample adds exactly nothing (no corner
 // of its cell is the voxel) and the sums are those of the full loop.  A view
 // whose panel is nearly edge-on to part of the box falls back to all pixels.
+// Parallel panels made from panel records project affinely along the beam;
+// explicit rays and cylindrical panels visit every pixel.
 #include "gather.h"
 
 #include <cmath>
@@ -48,6 +50,15 @@ __device__ __forceinline__ void grid_vector(const Params &P, const double *pose,
     double od[3];
     for (int i = 0; i < 3; ++i) od[i] = pose[i] * x[0] + pose[3 + i] * x[1] + pose[6 + i] * x[2];
     for (int i = 0; i < 3; ++i) out[i] = P.g[3 * i] * od[0] + P.g[3 * i + 1] * od[1] + P.g[3 * i + 2] * od[2];
+}
+
+// Grid coordinates of a world point under the view's pose (the ray-origin
+// arithmetic of kernels/projection.py:79-102)
+__device__ __forceinline__ void grid_point(const Params &P, const double *pose, const double *x, double *out) {
+    const double w[3] = {x[0] - pose[9], x[1] - pose[10], x[2] - pose[11]};
+    double rel[3];
+    for (int i = 0; i < 3; ++i) rel[i] = pose[i] * w[0] + pose[3 + i] * w[1] + pose[6 + i] * w[2] - P.go[i];
+    for (int i = 0; i < 3; ++i) out[i] = P.g[3 * i] * rel[0] + P.g[3 * i + 1] * rel[1] + P.g[3 * i + 2] * rel[2];
 }
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *out) {
@@ -86,16 +97,70 @@ __global__ __launch_bounds__(kBlock) void gather_checks_kernel(const double *__r
         }
 }
 
+// Parallel panel (segments generated from its record, geometry.py:349-354
+// of python/dpt): a grid point x lies on the segment line of pixel (c, r)
+// when x - E0 = c U + r V + t D (E0 = pixel (0,0) endpoint, U, V = column
+// and row steps, D = beam, all in grid coordinates), so
+// c = (x - E0).(V x D) / det and r = (x - E0).(D x U) / det with
+// det = U.(V x D): affine, its extremes over a box are at the corners.  A
+// beam within about 0.06 degrees of the panel plane (|det| below
+// kFacing |U| |V| |D|) visits every pixel.
+__device__ __forceinline__ void parallel_view(const Params &P, const double *pose, const double *m, int cols, int rows,
+                                              GatherView &gv) {
+    double e0[3], u[3], vv[3], d[3], uw[3], vw[3], dw[3];
+    for (int k = 0; k < 3; ++k) {
+        uw[k] = m[12] * m[6 + k];
+        vw[k] = m[13] * m[9 + k];
+        dw[k] = m[14] * m[k];
+    }
+    grid_point(P, pose, m + 3, e0);
+    grid_vector(P, pose, uw, u);
+    grid_vector(P, pose, vw, vv);
+    grid_vector(P, pose, dw, d);
+    double b[3], c[3], vd[3];
+    cross3(vv, d, vd);
+    cross3(d, u, c);
+    const double det = dot3(u, vd);
+    const double volume = sqrt(dot3(u, u)) * sqrt(dot3(vv, vv)) * sqrt(dot3(d, d));
+    if (!(isfinite(det) && isfinite(volume) && fabs(det) > kFacing * volume)) return;
+    for (int k = 0; k < 3; ++k) {
+        gv.s[k] = e0[k];
+        gv.b[k] = vd[k] / det;
+        gv.c[k] = c[k] / det;
+        b[k] = gv.b[k];
+    }
+    // the segment points' rounding scales with the world coordinates before
+    // the pose and grid maps as well as with the grid coordinates
+    double gmax = 0.0;
+    for (int k = 0; k < 9; ++k) gmax = fmax(gmax, fabs(P.g[k]));
+    const double world = max_abs3(m + 3) + max_abs3(pose + 9) + max_abs3(P.go) + m[14] * max_abs3(m) +
+                         (double)(cols - 1) * m[12] * max_abs3(m + 6) + (double)(rows - 1) * m[13] * max_abs3(m + 9);
+    const double reach = max_abs3(d) + (double)(cols - 1) * max_abs3(u) + (double)(rows - 1) * max_abs3(vv);
+    const double extent = (double)max(P.nx, max(P.ny, P.nz));
+    gv.margin = kBoxMargin * (1.0 + max_abs3(e0) + reach + 3.0 * gmax * world);
+    // |x - E0| <= |x| + |E0| over the widened boxes: bounds the rounding of
+    // the pixel coordinates together with that of b and c
+    gv.ratio = (sqrt(dot3(b, b)) + sqrt(dot3(gv.c, gv.c))) * (1.0 + extent + max_abs3(e0) + reach);
+    gv.norm_a = 1.0;
+    gv.affine = 1;
+    gv.ok = isfinite(gv.ratio) && isfinite(gv.margin);
+}
+
 // One thread per view: the panel projection used by the detector culling.
-__global__ void gather_views_kernel(Params P, GatherView *__restrict__ out) {
+__global__ void gather_views_kernel(Params P, const double *__restrict__ meta, const int *__restrict__ modes,
+                                    int image_rows, int image_cols, GatherView *__restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= P.views) return;
     GatherView gv;
     for (int i = 0; i < 3; ++i) gv.s[i] = gv.a[i] = gv.b[i] = gv.c[i] = 0.0;
     gv.norm_a = gv.ratio = gv.margin = 0.0;
     gv.ok = 0;
-    gv.pad = 0;
-    if (!P.bundle) {
+    gv.affine = 0;
+    gv.cols = P.bundle && meta != nullptr ? image_cols : P.cols;
+    gv.rows = P.bundle && meta != nullptr ? image_rows : P.rows;
+    if (P.bundle && meta != nullptr && modes[v] == 1) {
+        parallel_view(P, P.poses + 12 * (size_t)v, meta + 18 * (size_t)v, image_cols, image_rows, gv);
+    } else if (!P.bundle) {
         const double *pose = P.poses + 12 * (size_t)v;
         const double *pn = P.panels + 12 * (size_t)v;
         World w;
@@ -195,6 +260,29 @@ __device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &
     // Corner q of the widened box is w_lo + (q&1) e_x + (q>>1&1) e_y +
     // (q>>2&1) e_z relative to the source; its three dot products are sums
     // of per-axis terms (rounding differences are far inside the margins).
+    if (gv.affine) {
+        // parallel panel: affine pixel coordinates, no facing condition
+        double cmin = INFINITY, cmax = -INFINITY, rmin = INFINITY, rmax = -INFINITY;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double w[3];
+            for (int a = 0; a < 3; ++a)
+                w[a] = (((q >> a) & 1) ? vx.hi[a] + gv.margin : vx.lo[a] - gv.margin) - gv.s[a];
+            const double c = dot3(w, gv.b), r = dot3(w, gv.c);
+            cmin = fmin(cmin, c);
+            cmax = fmax(cmax, c);
+            rmin = fmin(rmin, r);
+            rmax = fmax(rmax, r);
+        }
+        if (!isfinite(cmin) || !isfinite(cmax) || !isfinite(rmin) || !isfinite(rmax)) return false;
+        const double mc = kPixelMargin * (1.0 + fmax(fabs(cmin), fabs(cmax)) + gv.ratio);
+        const double mr = kPixelMargin * (1.0 + fmax(fabs(rmin), fabs(rmax)) + gv.ratio);
+        c0 = max((int)ceil(fmin(fmax(cmin - mc, -1.0), (double)gv.cols)), 0);
+        c1 = min((int)floor(fmax(fmin(cmax + mc, (double)gv.cols), -1.0)), gv.cols - 1);
+        r0 = max((int)ceil(fmin(fmax(rmin - mr, -1.0), (double)gv.rows)), 0);
+        r1 = min((int)floor(fmax(fmin(rmax + mr, (double)gv.rows), -1.0)), gv.rows - 1);
+        return true;
+    }
     double den0 = 0.0, cn0 = 0.0, rn0 = 0.0, reach = 0.0;
     double da[3], db[3], dc[3];
 #pragma unroll
@@ -233,14 +321,14 @@ __device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &
     const double mc = kPixelMargin * (1.0 + fmax(fabs(cmin), fabs(cmax)) + gv.ratio);
     const double mr = kPixelMargin * (1.0 + fmax(fabs(rmin), fabs(rmax)) + gv.ratio);
     // clamped to [-1, n] before the integer conversion
-    c0 = (int)ceil(fmin(fmax(cmin - mc, -1.0), (double)P.cols));
-    c1 = (int)floor(fmax(fmin(cmax + mc, (double)P.cols), -1.0));
-    r0 = (int)ceil(fmin(fmax(rmin - mr, -1.0), (double)P.rows));
-    r1 = (int)floor(fmax(fmin(rmax + mr, (double)P.rows), -1.0));
+    c0 = (int)ceil(fmin(fmax(cmin - mc, -1.0), (double)gv.cols));
+    c1 = (int)floor(fmax(fmin(cmax + mc, (double)gv.cols), -1.0));
+    r0 = (int)ceil(fmin(fmax(rmin - mr, -1.0), (double)gv.rows));
+    r1 = (int)floor(fmax(fmin(rmax + mr, (double)gv.rows), -1.0));
     c0 = max(c0, 0);
     r0 = max(r0, 0);
-    c1 = min(c1, P.cols - 1);
-    r1 = min(r1, P.rows - 1);
+    c1 = min(c1, gv.cols - 1);
+    r1 = min(r1, gv.rows - 1);
     return true;
 }
 
@@ -386,20 +474,18 @@ __device__ __forceinline__ double gather_voxel(const Params &P, const GatherRay 
     const int64_t pixels = (int64_t)P.rows * P.cols;
     double total = 0.0;
     for (int view = 0; view < P.views; ++view) {
-        int c0 = 0, c1 = P.cols - 1, r0 = 0, r1 = P.rows - 1;
-        if (!P.bundle) {
-            const GatherView gv = views[view];
-            if (gv.ok && !pixel_window(P, gv, vx, c0, c1, r0, r1)) {
-                c0 = 0;
-                c1 = P.cols - 1;
-                r0 = 0;
-                r1 = P.rows - 1;
-            }
+        const GatherView gv = views[view];
+        int c0 = 0, c1 = gv.cols - 1, r0 = 0, r1 = gv.rows - 1;
+        if (gv.ok && !pixel_window(P, gv, vx, c0, c1, r0, r1)) {
+            c0 = 0;
+            c1 = gv.cols - 1;
+            r0 = 0;
+            r1 = gv.rows - 1;
         }
         const GatherRay *base = rays + (int64_t)view * pixels;
         for (int row = r0; row <= r1; ++row)
             for (int col = c0; col <= c1; ++col) {
-                const GatherRay g = load_ray(base + (int64_t)row * P.cols + col);
+                const GatherRay g = load_ray(base + (int64_t)row * gv.cols + col);
                 int k0, k1;
                 const bool reach = sample_window(P, g, vx, k0, k1);
                 if (g.vf != 0.0 && reach) total = add_ray<MODE>(P, g, vx, k0, k1, total);
@@ -477,13 +563,16 @@ void launch_voxels(const Params &P, const GatherRay *rays, const GatherView *vie
 }  // namespace
 
 hipError_t launch_gather(const Params &P, const double *cotangent, GatherRay *rays, GatherView *views, int *guard,
-                         void *out, bool out_fp64, bool accumulate, bool checks, hipStream_t s) {
+                         void *out, bool out_fp64, bool accumulate, bool checks, const double *panel_meta,
+                         const int *panel_modes, int image_cols, hipStream_t s) {
     const int64_t pixels = (int64_t)P.views * P.rows * P.cols;
     hipLaunchKernelGGL(clear_guard_kernel, dim3(1), dim3(1), 0, s, guard);
     if (checks)
         hipLaunchKernelGGL(gather_checks_kernel, dim3(stride_blocks(pixels > P.views ? pixels : P.views)),
                            dim3(kBlock), 0, s, P.poses, P.views, cotangent, pixels, P.status, guard);
-    hipLaunchKernelGGL(gather_views_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, P, views);
+    const int image_rows = panel_meta != nullptr ? (int)(P.cols / image_cols) : P.rows;
+    hipLaunchKernelGGL(gather_views_kernel, dim3((P.views + 63) / 64), dim3(64), 0, s, P, panel_meta, panel_modes,
+                       image_rows, image_cols, views);
     hipLaunchKernelGGL(gather_rays_kernel, dim3(stride_blocks(pixels)), dim3(kBlock), 0, s, P, cotangent, rays);
     if (out_fp64)
         launch_voxels<double>(P, rays, views, guard, static_cast<double *>(out), accumulate ? 1 : 0, s);

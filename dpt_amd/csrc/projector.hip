@@ -1589,6 +1589,9 @@ struct dpt_plan {
     GatherRay *d_gather_rays = nullptr;    // adjoint gather workspace, allocated at its first call
     GatherView *d_gather_views = nullptr;
     int *d_guard = nullptr;
+    double *d_panel_meta = nullptr;  // panel records of a plan made from them (the gather's parallel-panel windows)
+    int *d_panel_modes = nullptr;
+    int image_cols = 0;              // detector columns of such a plan
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool timed[3] = {false, false, false};
 };
@@ -1851,6 +1854,8 @@ int dpt_plan_destroy(dpt_plan *p) {
     (void)hipFree(p->d_gather_rays);
     (void)hipFree(p->d_gather_views);
     (void)hipFree(p->d_guard);
+    (void)hipFree(p->d_panel_meta);
+    (void)hipFree(p->d_panel_modes);
     if (p->h_status) (void)hipHostFree(p->h_status);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1998,22 +2003,18 @@ int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t 
         ALLOC(p->d_ray_src, bytes);
         ALLOC(p->d_ray_end, bytes);
         if (generated) {
-            double *d_meta = nullptr;
-            int *d_modes = nullptr;
-            ALLOC(d_meta, sizeof(double) * 18 * views);
-            if (hipMalloc((void **)&d_modes, sizeof(int) * views) != hipSuccess) {
-                (void)hipFree(d_meta);
-                return cleanup(DPT_E_MEMORY, "hipMalloc failed for the panel modes");
-            }
-            bool ok = hipMemcpy(d_meta, panel_meta, sizeof(double) * 18 * views, hipMemcpyHostToDevice) == hipSuccess &&
-                      hipMemcpy(d_modes, panel_modes, sizeof(int) * views, hipMemcpyHostToDevice) == hipSuccess;
+            ALLOC(p->d_panel_meta, sizeof(double) * 18 * views);
+            ALLOC(p->d_panel_modes, sizeof(int) * views);
+            p->image_cols = image_cols;
+            bool ok = hipMemcpy(p->d_panel_meta, panel_meta, sizeof(double) * 18 * views, hipMemcpyHostToDevice) ==
+                          hipSuccess &&
+                      hipMemcpy(p->d_panel_modes, panel_modes, sizeof(int) * views, hipMemcpyHostToDevice) ==
+                          hipSuccess;
             if (ok) {
-                hipLaunchKernelGGL(panel_rays_kernel, dim3(grid_blocks(pixels)), dim3(kBlock), 0, 0, d_meta, d_modes,
-                                   (int64_t)cols, image_cols, pixels, p->d_ray_src, p->d_ray_end);
+                hipLaunchKernelGGL(panel_rays_kernel, dim3(grid_blocks(pixels)), dim3(kBlock), 0, 0, p->d_panel_meta,
+                                   p->d_panel_modes, (int64_t)cols, image_cols, pixels, p->d_ray_src, p->d_ray_end);
                 ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
             }
-            (void)hipFree(d_meta);
-            (void)hipFree(d_modes);
             if (!ok) return cleanup(DPT_E_DEVICE, "panel ray generation failed");
         } else if (hipMemcpy(p->d_ray_src, ray_sources, bytes, hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(p->d_ray_end, ray_endpoints, bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -2311,7 +2312,7 @@ int dpt_adjoint_gather(dpt_plan *p, const double *poses, const double *cotangent
     timer_begin(p, 2, s);
     HIP_TRY(launch_gather(P, cotangent, p->d_gather_rays, p->d_gather_views, p->d_guard, out,
                           out_precision == DPT_ARITH_FP64, (flags & DPT_ACCUMULATE) != 0,
-                          (flags & DPT_NO_CHECKS) == 0, s));
+                          (flags & DPT_NO_CHECKS) == 0, p->d_panel_meta, p->d_panel_modes, p->image_cols, s));
     timer_end(p, 2, s);
     return DPT_OK;
 }
