@@ -11,7 +11,17 @@
  *   FP64 sampling  kernels/projection.py       (forward :802-846, VJP :873-1237)
  *
  * with the FP64 ray setup of kernels/projection.py:79-121 and the pose algebra
- * of kernels/geometry.py:25-52.  Build with -ffp-contract=off: every fused
+ * of kernels/geometry.py:25-52, and three more families:
+ *
+ *   deterministic gather  kernels/projection_adjoint.py:73-170 (serial scatter
+ *                         in (view, pixel, sample, corner) order: ref_gather_f64)
+ *   exact cell-Gauss      kernels/projection.py:522-746 (ref_forward_gauss_f64,
+ *                         ref_vjp_gauss_f64)
+ *   ray coordinates       per-ray world source / endpoint cotangents (output
+ *                         mode 7 of the VJPs), kernels/projection_fp32.py:570-583
+ *                         and kernels/projection.py:1184-1200
+ *
+ * Build with -ffp-contract=off: every fused
  * multiply-add below is an explicit fmaf/fma, so unmarked products never fuse.
  * Where the reference leaves an expression unmarked but its code generator
  * (Warp, fuse_fp=True) contracts it, this file writes the contracted form
