@@ -58,6 +58,8 @@ def _args():
     parser.add_argument("--no-cell-gauss", action="store_true", help="skip the exact cell-Gauss measurement")
     parser.add_argument("--cell-gauss-only", action="store_true",
                         help="only the exact cell-Gauss measurement (profiling driver)")
+    parser.add_argument("--gather-chest-only", action="store_true",
+                        help="only the deterministic gather on the chest workload (one JSON object)")
     parser.add_argument("--gather-only", action="store_true",
                         help="only the deterministic adjoint gather measurement (profiling driver)")
     parser.add_argument("--transport-only", action="store_true",
@@ -423,6 +425,92 @@ def gather_benchmark(grid, geometry, values, target, samples: int) -> dict:
     return result
 
 
+# The reference's own gather workload (benchmarks/gather.py:49-160 of the
+# reference, results docs/assets/results/extra-gather-chest-512-valid02.json):
+# a 512 x 512 x 139 chest grid (0.762 x 0.762 x 2.5 mm), its AP view on a
+# 512^2 flat panel at 2.674 mm pitch, 1024 FP64 midpoint samples, identity
+# pose, and two unit-sum cotangents; device medians on the reference's GPU.
+CHEST_GRID = dict(shape=(139, 512, 512), spacing_mm=(0.7617189884185791, 0.7617189884185791, 2.5),
+                  origin_mm=(195.0, 171.6999969482422, -347.75),
+                  orientation=(-1.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0, 0.0, 1.0))
+CHEST_AP = dict(source_mm=(0.3807984590531992, 1280.854457404986, -175.25),
+                origin_mm=(683.5250937177063, -674.8060355915503, -858.3942952586534),
+                u=(-1.0, 1.2246467991473532e-16, 0.0), v=(0.0, 0.0, 1.0),
+                spacing_mm=(2.673754580268702, 2.673754580268702), shape=(512, 512))
+CHEST_REFERENCE_MS = {"detector_region": 101.58100509643555, "full_detector_mean": 173.40975952148438}
+CHEST_REFERENCE_SCATTER_MS = {"detector_region": 3.5854079723358154, "full_detector_mean": 5.735919952392578}
+
+
+def gather_chest_benchmark(repeats: int = 10) -> dict:
+    """Deterministic gather on the reference's chest workload.  The gather
+    depends only on the geometry and the cotangent, so the field (a CT in
+    the reference's run) does not enter.  Patterns: every pixel 1/N
+    ("full_detector_mean") and the central 6/16 x 6/16 window normalised to
+    unit sum ("detector_region").  Device time from HIP events on the launch
+    stream, median of ``repeats`` calls after one warm call.  The ray-driven
+    FP64 volume VJP of the same cotangent (the reference's "scatter") is
+    timed beside it with torch events around the backward call: accumulator
+    clear, VJP kernel and the FP64-to-FP32 gradient conversion."""
+    import numpy as np
+    import torch
+
+    from dpt_amd.geometry import DetectorGeometry
+    from dpt_amd.projection import prepare
+    from dpt_amd.torch import Projector
+    from dpt_amd.volumes import GridSpec
+
+    device = torch.device("cuda:0")
+    grid = GridSpec(**CHEST_GRID)
+    geometry = DetectorGeometry(**CHEST_AP)
+    module = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=1024)
+    pose = torch.zeros(6, dtype=torch.float64, device=device)
+    rows, cols = CHEST_AP["shape"]
+    result = {"grid": "512 x 512 x 139 (0.762 x 0.762 x 2.5 mm)", "detector": "512 x 512, 2.674 mm, AP view",
+              "samples": 1024, "output": "float64"}
+    for pattern in ("detector_region", "full_detector_mean"):
+        weights = torch.ones((1, rows, cols), dtype=torch.float64, device=device)
+        if pattern == "detector_region":
+            weights.zero_()
+            weights[:, rows * 5 // 16: rows * 11 // 16, cols * 5 // 16: cols * 11 // 16] = 1.0
+        weights /= weights.sum()
+        out = module.adjoint_gather(weights, pose)  # warm call: allocates the plan's ray table
+        entry = prepare(module._configuration, 0, torch.cuda.current_stream(device).cuda_stream, 0)
+        entry.plan.set_timing(True)
+        first = out.clone()
+        kernel = []
+        for _ in range(repeats):
+            out = module.adjoint_gather(weights, pose)
+            torch.cuda.synchronize()
+            kernel.append(entry.plan.kernel_ms(2))
+        ms = float(np.median(kernel))
+        # the ray-driven FP64 volume VJP of the same cotangent (linear in the
+        # field, so a zero field serves)
+        leaf = torch.zeros(CHEST_GRID["shape"], dtype=torch.float32, device=device, requires_grad=True)
+        (atomic,) = torch.autograd.grad(module(leaf, pose), leaf, weights)
+        scatter = []
+        for _ in range(repeats):
+            image = module(leaf, pose)
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            (atomic,) = torch.autograd.grad(image, leaf, weights)
+            stop.record()
+            torch.cuda.synchronize()
+            scatter.append(start.elapsed_time(stop))
+        a, b = out.reshape(-1), atomic.double().reshape(-1)
+        rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+        result[pattern] = {"device_median_ms": round(ms, 3), "bitwise_repeat": bool(torch.equal(first, out)),
+                           "rel_l2_vs_atomic_fp64_volume_vjp": rel,
+                           "atomic_volume_vjp_ms": round(float(np.median(scatter)), 3),
+                           "reference_scatter_device_median_ms": CHEST_REFERENCE_SCATTER_MS[pattern],
+                           "nonzero_voxels": int(torch.count_nonzero(out)),
+                           "reference_device_median_ms": CHEST_REFERENCE_MS[pattern],
+                           "speedup_vs_reference": round(CHEST_REFERENCE_MS[pattern] / ms, 2)}
+        _log(f"chest gather {pattern}: {ms:.3f} ms (reference {CHEST_REFERENCE_MS[pattern]:.1f} ms), "
+             f"rel vs atomic {rel:.2e}")
+        del out, first, atomic, leaf
+    return result
+
+
 def cell_gauss_benchmark(grid, geometry, values, target, samples: int) -> dict:
     """Exact cell-Gauss integration of the full scan (FP64 sampling, two Gauss
     nodes per interpolation cell): forward, forward + pose VJP and forward +
@@ -624,6 +712,9 @@ def main() -> int:
         print(json.dumps(transport_benchmark(args.transport_histories, args.transport_cpu_histories,
                                              not args.no_cpu_baseline)), flush=True)
         return 0
+    if args.gather_chest_only:
+        print(json.dumps(gather_chest_benchmark()), flush=True)
+        return 0
     import numpy as np
     import torch
 
@@ -814,6 +905,7 @@ def main() -> int:
     if not args.no_gather:
         _log("deterministic adjoint gather")
         result["gather"] = gather_benchmark(grid, geometry, values, target, args.samples)
+        result["gather_chest"] = gather_chest_benchmark()
     if not args.no_transport:
         _log("transport benchmark")
         result["transport"] = transport_benchmark(args.transport_histories, args.transport_cpu_histories,

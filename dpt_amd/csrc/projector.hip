@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 #include <string>
@@ -288,6 +289,72 @@ inline int chunk_samples(int samples, int nx, int ny, int nz) {
 // cycles per wave64 instruction, measured), so the brick accumulates the FP32
 // cell sums in FP64.
 constexpr int kBrickCapDefault = 4096;
+
+// Host estimate of the chunk bricks of the first view at the identity pose:
+// the grid samples touched by 16 x 16 pixel tiles (every 4th tile along
+// each detector axis, at most 16 per axis) over one chunk of samples at a
+// quarter, half and three quarters of the ray.  A plan takes bricks of
+// 2 * kBrickCapDefault FP64 entries (64 KiB of LDS, fewer workgroups per CU)
+// only when at least a quarter of the sampled bricks overflow
+// kBrickCapDefault yet fit the larger brick: magnified detectors over fine
+// grids, whose tiles cover wide footprints and would otherwise fall back to
+// device atomics.  (A 256^3 grid under a 512^2 panel at 1.5x magnification
+// overflows in 9 % of the samples and keeps the default; a 512 x 512 x 139
+// chest grid at 0.76 mm under a 512^2 panel at 2.67 mm overflows in 60 %.)
+int choose_brick_cap(const Params &P, const double *panels) {
+    const int n[3] = {P.nx, P.ny, P.nz};
+    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
+    const int tile = 16;
+    const int tiles_y = (P.rows + tile - 1) / tile, tiles_x = (P.cols + tile - 1) / tile;
+    const int step_y = std::max(4, (tiles_y + 15) / 16), step_x = std::max(4, (tiles_x + 15) / 16);
+    const int sampled_chunks[3] = {chunks / 4, chunks / 2, (3 * chunks) / 4};
+    int measured = 0, rescued = 0;
+    for (int ty = 0; ty < tiles_y; ty += step_y)
+        for (int tx = 0; tx < tiles_x; tx += step_x)
+            for (int q = 0; q < 3; ++q) {
+                const int k0 = sampled_chunks[q] * P.chunk_samples, k1 = std::min(k0 + P.chunk_samples, P.samples);
+                int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+                for (int r = ty * tile; r < std::min(ty * tile + tile, P.rows); ++r)
+                    for (int c = tx * tile; c < std::min(tx * tile + tile, P.cols); ++c) {
+                        double src[3], end[3], o[3], d[3];
+                        for (int a = 0; a < 3; ++a) {
+                            src[a] = panels[a] - P.go[a];
+                            end[a] = panels[3 + a] + c * panels[6 + a] + r * panels[9 + a] - panels[a];
+                        }
+                        for (int a = 0; a < 3; ++a) {
+                            o[a] = P.g[3 * a] * src[0] + P.g[3 * a + 1] * src[1] + P.g[3 * a + 2] * src[2];
+                            d[a] = P.g[3 * a] * end[0] + P.g[3 * a + 1] * end[1] + P.g[3 * a + 2] * end[2];
+                        }
+                        double t0 = 0.0, t1 = 1.0;
+                        for (int a = 0; a < 3; ++a) {
+                            if (d[a] == 0.0) {
+                                if (o[a] < -0.5 || o[a] > n[a] - 0.5) t1 = -1.0;
+                                continue;
+                            }
+                            const double u = (-0.5 - o[a]) / d[a], v = (n[a] - 0.5 - o[a]) / d[a];
+                            t0 = std::max(t0, std::min(u, v));
+                            t1 = std::min(t1, std::max(u, v));
+                        }
+                        if (!(t1 > t0)) continue;
+                        const double step = (t1 - t0) / P.samples;
+                        for (int k : {k0, k1 - 1}) {
+                            const double pos = t0 + (k + 0.5) * step;
+                            for (int a = 0; a < 3; ++a) {
+                                const double x = std::min(std::max(o[a] + pos * d[a], 0.0), (double)(n[a] - 1));
+                                const int l = std::min((int)std::floor(x), std::max(n[a] - 2, 0));
+                                lo[a] = std::min(lo[a], l);
+                                hi[a] = std::max(hi[a], l);
+                            }
+                        }
+                    }
+                if (lo[0] == INT_MAX) continue;
+                long long volume = 1;
+                for (int a = 0; a < 3; ++a) volume *= std::min(hi[a] + 1, n[a] - 1) - lo[a] + 1;
+                ++measured;
+                rescued += volume > kBrickCapDefault && volume <= 2 * kBrickCapDefault;
+            }
+    return measured > 0 && 4 * rescued >= measured ? 2 * kBrickCapDefault : kBrickCapDefault;
+}
 
 // Clamped lower cell coordinate of one FP32 sample (same arithmetic as cell32,
 // without the support test: out-of-support samples only widen the box).
@@ -1954,12 +2021,12 @@ int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t 
     P.samples = samples;
     P.bundle = panels == nullptr ? 1 : 0;
     configure_tiles(P, 1);
-    P.brick_cap = kBrickCapDefault;
     P.chunk_samples = chunk_samples(samples, grid->nx, grid->ny, grid->nz);
     if (const char *chunk = std::getenv("DPT_AMD_CHUNK")) {
         const int forced = std::atoi(chunk);
         if (forced >= 1 && forced <= 1024) P.chunk_samples = forced;
     }
+    P.brick_cap = P.bundle ? kBrickCapDefault : choose_brick_cap(P, panels);
     if (const char *cap = std::getenv("DPT_AMD_BRICK_CAP")) {
         const int forced = std::atoi(cap);
         if (forced >= 512 && forced <= 8192) P.brick_cap = forced;
