@@ -22,8 +22,9 @@
 // the sample points, so every skipped sample adds exactly nothing (no corner
 // of its cell is the voxel) and the sums are those of the full loop.  A view
 // whose panel is nearly edge-on to part of the box falls back to all pixels.
-// Parallel panels made from panel records project affinely along the beam;
-// explicit rays and cylindrical panels visit every pixel.
+// Panel records: flat cone panels project as above and parallel panels
+// affinely along the beam; explicit rays and cylindrical panels visit every
+// pixel.
 #include "gather.h"
 
 #include <cmath>
@@ -146,6 +147,45 @@ __device__ __forceinline__ void parallel_view(const Params &P, const double *pos
     gv.ok = isfinite(gv.ratio) && isfinite(gv.margin);
 }
 
+// Flat cone panel pn (source, pixel (0,0) centre, column step, row step;
+// world coordinates): the projective pixel coordinates through the source.
+__device__ __forceinline__ void cone_view(const Params &P, const double *pose, const double *pn, int cols, int rows,
+                                          GatherView &gv) {
+    World w;
+    for (int i = 0; i < 3; ++i) {
+        w.wsr[i] = pn[i] - pose[9 + i];
+        w.wd[i] = pn[3 + i] - pn[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        w.os[i] = pose[i] * w.wsr[0] + pose[3 + i] * w.wsr[1] + pose[6 + i] * w.wsr[2];
+        w.od[i] = pose[i] * w.wd[0] + pose[3 + i] * w.wd[1] + pose[6 + i] * w.wd[2];
+    }
+    Ray r;
+    ray_grid(P, w, r);
+    double u[3], vv[3], a[3], b[3], c[3];
+    grid_vector(P, pose, pn + 6, u);
+    grid_vector(P, pose, pn + 9, vv);
+    cross3(u, vv, a);
+    cross3(vv, r.d, b);
+    cross3(r.d, u, c);
+    const double det = dot3(r.d, a);
+    if (r.valid && isfinite(det) && det != 0.0) {
+        const double sign = det > 0.0 ? 1.0 : -1.0;
+        for (int i = 0; i < 3; ++i) {
+            gv.s[i] = r.o[i];
+            gv.a[i] = sign * a[i];
+            gv.b[i] = sign * b[i];
+            gv.c[i] = sign * c[i];
+        }
+        gv.norm_a = sqrt(dot3(a, a));
+        gv.ratio = (sqrt(dot3(b, b)) + sqrt(dot3(c, c))) / gv.norm_a;
+        const double reach = max_abs3(r.d) + (double)(cols - 1) * max_abs3(u) + (double)(rows - 1) * max_abs3(vv);
+        gv.margin = kBoxMargin * (1.0 + max_abs3(r.o) + reach);
+        gv.ok = isfinite(gv.ratio) && isfinite(gv.margin) && gv.norm_a > 0.0 && isfinite(dot3(gv.b, gv.b)) &&
+                isfinite(dot3(gv.c, gv.c));
+    }
+}
+
 // One thread per view: the panel projection used by the detector culling.
 __global__ void gather_views_kernel(Params P, const double *__restrict__ meta, const int *__restrict__ modes,
                                     int image_rows, int image_cols, GatherView *__restrict__ out) {
@@ -160,35 +200,19 @@ __global__ void gather_views_kernel(Params P, const double *__restrict__ meta, c
     gv.rows = P.bundle && meta != nullptr ? image_rows : P.rows;
     if (P.bundle && meta != nullptr && modes[v] == 1) {
         parallel_view(P, P.poses + 12 * (size_t)v, meta + 18 * (size_t)v, image_cols, image_rows, gv);
-    } else if (!P.bundle) {
-        const double *pose = P.poses + 12 * (size_t)v;
-        const double *pn = P.panels + 12 * (size_t)v;
-        World w;
-        ray_world(P, pose, v, 0, 0, w);
-        Ray r;
-        ray_grid(P, w, r);
-        double u[3], vv[3], a[3], b[3], c[3];
-        grid_vector(P, pose, pn + 6, u);
-        grid_vector(P, pose, pn + 9, vv);
-        cross3(u, vv, a);
-        cross3(vv, r.d, b);
-        cross3(r.d, u, c);
-        const double det = dot3(r.d, a);
-        if (r.valid && isfinite(det) && det != 0.0) {
-            const double sign = det > 0.0 ? 1.0 : -1.0;
-            for (int i = 0; i < 3; ++i) {
-                gv.s[i] = r.o[i];
-                gv.a[i] = sign * a[i];
-                gv.b[i] = sign * b[i];
-                gv.c[i] = sign * c[i];
-            }
-            gv.norm_a = sqrt(dot3(a, a));
-            gv.ratio = (sqrt(dot3(b, b)) + sqrt(dot3(c, c))) / gv.norm_a;
-            const double reach = max_abs3(r.d) + (double)(P.cols - 1) * max_abs3(u) + (double)(P.rows - 1) * max_abs3(vv);
-            gv.margin = kBoxMargin * (1.0 + max_abs3(r.o) + reach);
-            gv.ok = isfinite(gv.ratio) && isfinite(gv.margin) && gv.norm_a > 0.0 &&
-                    isfinite(dot3(gv.b, gv.b)) && isfinite(dot3(gv.c, gv.c));
+    } else if (P.bundle && meta != nullptr && modes[v] == 0) {
+        // flat cone panel from its record (source, pixel (0,0), u, v, pitches)
+        const double *m = meta + 18 * (size_t)v;
+        double pn[12];
+        for (int k = 0; k < 3; ++k) {
+            pn[k] = m[k];
+            pn[3 + k] = m[3 + k];
+            pn[6 + k] = m[12] * m[6 + k];
+            pn[9 + k] = m[13] * m[9 + k];
         }
+        cone_view(P, P.poses + 12 * (size_t)v, pn, image_cols, image_rows, gv);
+    } else if (!P.bundle) {
+        cone_view(P, P.poses + 12 * (size_t)v, P.panels + 12 * (size_t)v, P.cols, P.rows, gv);
     }
     out[v] = gv;
 }

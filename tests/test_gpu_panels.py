@@ -102,10 +102,11 @@ def test_parallel_panel_of_a_constant_field_measures_chords():
 
 def test_gather_on_mixed_panels_is_bitwise_the_oracle():
     """The deterministic gather over device-generated segments of cone,
-    parallel and cylindrical panels (no projected-window culling for these
-    views) repeats bit for bit and equals the oracle's gather over the
-    independently generated segments to 1e-12 (the cylindrical endpoints use
-    the device's cos/sin, which may differ from the host's in the last bit)."""
+    parallel and cylindrical panels (projected pixel windows for the cone and
+    parallel views, every pixel for the cylindrical one) repeats bit for bit
+    and equals the oracle's gather over the independently generated segments
+    to 1e-12 (the cylindrical endpoints use the device's cos/sin, which may
+    differ from the host's in the last bit)."""
     grid, spec, geometry, ref, samples = _setup(samples=31)
     module = Projector(grid=spec, geometry=geometry, samples_per_ray=samples, precision="fp64")
     seeds = np.random.default_rng(4).uniform(-1.0, 1.0, 3 * 9 * 11)
@@ -173,3 +174,54 @@ def test_gather_on_parallel_panels_is_bitwise_the_oracle():
     assert out.cpu().numpy().reshape(-1).tobytes() == expected.tobytes()
     single = module.adjoint_gather(cot, packed_pose=pose, volume_gradient_precision="float32")
     assert single.cpu().numpy().reshape(-1).tobytes() == expected.astype(np.float32).tobytes()
+
+
+def test_gather_on_cone_panel_records_matches_the_oracle():
+    """Flat cone panels inside a mixed batch reach the gather as panel
+    records and take the projective pixel windows; two cone views at
+    different angles beside a parallel one, off-identity pose: equal to the
+    oracle's serial scatter over host-generated segments to 1e-12 per voxel
+    (a dropped window would miss whole rays, far above that), bit-identical
+    on repeat."""
+    print("case gather cone panel records", flush=True)
+    grid = O.Grid((14, 16, 15), (1.0, 0.9, 1.1), (-6.5, -6.75, -7.7))
+    cone, parallel, _ = scenes.mixed_panels(grid, rows=12, cols=13)
+    rows, cols = cone.shape
+    centre = [grid.origin[i] + ((grid.xyz[i] - 1) / 2) * grid.spacing[i] for i in range(3)]
+    a = 0.9  # second cone panel: the first rotated about the vertical axis through the centre
+    def turn(p):
+        x, y = p[0] - centre[0], p[1] - centre[1]
+        return (centre[0] + math.cos(a) * x - math.sin(a) * y, centre[1] + math.sin(a) * x + math.cos(a) * y, p[2])
+    u2 = (math.cos(a) * cone.u[0] - math.sin(a) * cone.u[1], math.sin(a) * cone.u[0] + math.cos(a) * cone.u[1], cone.u[2])
+    cone2 = O.Panel(turn(cone.source), turn(cone.origin), u2, cone.v, cone.spacing, cone.shape)
+    panels = (DetectorGeometry(cone.source, cone.origin, cone.u, cone.v, cone.spacing, cone.shape),
+              DetectorGeometry(cone2.source, cone2.origin, cone2.u, cone2.v, cone2.spacing, cone2.shape),
+              ParallelGeometry(**parallel))
+    sources, endpoints = [], []
+    for view in range(3):
+        src, end = [], []
+        for r in range(rows):
+            for c in range(cols):
+                if view < 2:
+                    p = (cone, cone2)[view]
+                    s_, e_ = p.source, p.pixel(r, c)
+                else:
+                    s_, e_ = scenes.parallel_segment(parallel, r, c)
+                src.append(s_)
+                end.append(e_)
+        sources.append(src)
+        endpoints.append(end)
+    samples = 29
+    ref = O.Scene(grid, samples, rays=(np.asarray(sources), np.asarray(endpoints)))
+    spec = GridSpec(grid.shape, grid.spacing, grid.origin, grid.orientation)
+    module = Projector(grid=spec, geometry=BatchedGeometry(panels), samples_per_ray=samples, precision="fp64")
+    seeds = np.random.default_rng(8).uniform(-1.0, 1.0, 3 * rows * cols)
+    packed = np.asarray([scenes.compose(scenes.IDENTITY12, (-0.2, 0.1, 0.25, 0.015, 0.02, -0.02))] * 3)
+    cot = torch.tensor(seeds.reshape(3, rows, cols), device="cuda")
+    pose = torch.tensor(packed, device="cuda")
+    out = module.adjoint_gather(cot, packed_pose=pose)
+    assert torch.equal(out, module.adjoint_gather(cot, packed_pose=pose))
+    expected, status = ref.gather(seeds, packed)
+    assert status.tolist() == [0, 0, 0] and np.count_nonzero(expected) > 0.5 * expected.size
+    np.testing.assert_allclose(out.cpu().numpy().reshape(-1), expected, rtol=1e-12,
+                               atol=1e-14 * np.abs(expected).max())
