@@ -45,6 +45,7 @@ struct Params {
     const double *poses;     // views x 12
     int *status;             // views
     int brick_cap;           // FP64 LDS brick entries per workgroup (volume adjoint)
+    int slab_chunks;         // 1: the FP64 volume adjoint chunks by slabs of space, not by sample index
     int chunk_samples;       // samples per lockstep chunk of the volume adjoint
     const float *alpha;      // 2 x samples entries: alpha_k, then k + 0.5 (sample_table_entries)
 };

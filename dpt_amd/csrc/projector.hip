@@ -301,16 +301,103 @@ constexpr int kBrickCapDefault = 4096;
 // device atomics.  (A 256^3 grid under a 512^2 panel at 1.5x magnification
 // overflows in 9 % of the samples and keeps the default; a 512 x 512 x 139
 // chest grid at 0.76 mm under a 512^2 panel at 2.67 mm overflows in 60 %.)
-int choose_brick_cap(const Params &P, const double *panels) {
+// The same samples decide whether the FP64 volume adjoint chunks by slabs
+// (*slab_chunks): the chest grid's tiles at the silhouette overflow even the
+// larger brick in 15 % of the index chunks and in none of the slab chunks.
+int choose_brick_cap(const Params &P, const double *panels, int *slab_chunks) {
     const int n[3] = {P.nx, P.ny, P.nz};
     const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
     const int tile = 16;
     const int tiles_y = (P.rows + tile - 1) / tile, tiles_x = (P.cols + tile - 1) / tile;
     const int step_y = std::max(4, (tiles_y + 15) / 16), step_x = std::max(4, (tiles_x + 15) / 16);
     const int sampled_chunks[3] = {chunks / 4, chunks / 2, (3 * chunks) / 4};
-    int measured = 0, rescued = 0;
+    int measured = 0, rescued = 0, index_over = 0, slab_measured = 0, slab_over = 0;  // over: > 2 * default
     for (int ty = 0; ty < tiles_y; ty += step_y)
-        for (int tx = 0; tx < tiles_x; tx += step_x)
+        for (int tx = 0; tx < tiles_x; tx += step_x) {
+            // the tile's rays in grid coordinates with their clipped intervals
+            struct TileRay {
+                double o[3], d[3], t0, step;
+            };
+            std::vector<TileRay> rays;
+            for (int r = ty * tile; r < std::min(ty * tile + tile, P.rows); ++r)
+                for (int c = tx * tile; c < std::min(tx * tile + tile, P.cols); ++c) {
+                    TileRay tr;
+                    double src[3], end[3];
+                    for (int a = 0; a < 3; ++a) {
+                        src[a] = panels[a] - P.go[a];
+                        end[a] = panels[3 + a] + c * panels[6 + a] + r * panels[9 + a] - panels[a];
+                    }
+                    for (int a = 0; a < 3; ++a) {
+                        tr.o[a] = P.g[3 * a] * src[0] + P.g[3 * a + 1] * src[1] + P.g[3 * a + 2] * src[2];
+                        tr.d[a] = P.g[3 * a] * end[0] + P.g[3 * a + 1] * end[1] + P.g[3 * a + 2] * end[2];
+                    }
+                    double t0 = 0.0, t1 = 1.0;
+                    for (int a = 0; a < 3; ++a) {
+                        if (tr.d[a] == 0.0) {
+                            if (tr.o[a] < -0.5 || tr.o[a] > n[a] - 0.5) t1 = -1.0;
+                            continue;
+                        }
+                        const double u = (-0.5 - tr.o[a]) / tr.d[a], v = (n[a] - 0.5 - tr.o[a]) / tr.d[a];
+                        t0 = std::max(t0, std::min(u, v));
+                        t1 = std::min(t1, std::max(u, v));
+                    }
+                    if (!(t1 > t0)) continue;
+                    tr.t0 = t0;
+                    tr.step = (t1 - t0) / P.samples;
+                    rays.push_back(tr);
+                }
+            if (rays.empty()) continue;
+            // slab chunks (the device's slab_setup / slab_range, in FP64 on the host)
+            int votes[3] = {0, 0, 0};
+            for (const TileRay &tr : rays) {
+                const double ad[3] = {std::fabs(tr.d[0]), std::fabs(tr.d[1]), std::fabs(tr.d[2])};
+                ++votes[ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2)];
+            }
+            const int A = votes[0] >= votes[1] && votes[0] >= votes[2] ? 0 : (votes[1] >= votes[2] ? 1 : 2);
+            int slo = INT_MAX, shi = INT_MIN, adv = 1;
+            for (const TileRay &tr : rays) {
+                const double q0 = tr.o[A] + (tr.t0 + 0.5 * tr.step) * tr.d[A];
+                const double q1 = tr.o[A] + (tr.t0 + (P.samples - 0.5) * tr.step) * tr.d[A];
+                slo = std::min(slo, (int)std::floor(std::min(std::max(std::min(q0, q1), -2.0), n[A] + 1.0)));
+                shi = std::max(shi, (int)std::ceil(std::min(std::max(std::max(q0, q1), -2.0), n[A] + 1.0)));
+                adv = std::max(adv, (int)std::min(std::ceil(tr.step * std::fabs(tr.d[A]) * 64.0), 1048576.0));
+            }
+            const int width = std::max(1, (P.chunk_samples * adv + 63) / 64);
+            const int count = std::max(1, (shi - slo + width - 1) / width);
+            for (int j : {count / 4, count / 2, (3 * count) / 4}) {
+                int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+                for (const TileRay &tr : rays) {
+                    const double da = tr.d[A];
+                    if (da == 0.0) continue;
+                    const int js = da > 0.0 ? j : count - 1 - j;
+                    const double b0 = slo + js * width, b1 = slo + (js + 1) * width;
+                    const double t0 = ((b0 - tr.o[A]) / da - tr.t0) / tr.step - 0.5;
+                    const double t1 = ((b1 - tr.o[A]) / da - tr.t0) / tr.step - 0.5;
+                    double k0, k1;
+                    if (da > 0.0) {
+                        k0 = j == 0 ? 0.0 : std::min(std::max(std::ceil(t0), 0.0), (double)P.samples);
+                        k1 = j == count - 1 ? P.samples : std::min(std::max(std::ceil(t1), 0.0), (double)P.samples);
+                    } else {
+                        k0 = j == 0 ? 0.0 : std::min(std::max(std::floor(t1) + 1.0, 0.0), (double)P.samples);
+                        k1 = j == count - 1 ? P.samples : std::min(std::max(std::floor(t0) + 1.0, 0.0), (double)P.samples);
+                    }
+                    if (!(k1 > k0)) continue;
+                    for (double k : {k0, k1 - 1.0}) {
+                        const double pos = tr.t0 + (k + 0.5) * tr.step;
+                        for (int a = 0; a < 3; ++a) {
+                            const double x = std::min(std::max(tr.o[a] + pos * tr.d[a], 0.0), (double)(n[a] - 1));
+                            const int l = std::min((int)std::floor(x), std::max(n[a] - 2, 0));
+                            lo[a] = std::min(lo[a], l);
+                            hi[a] = std::max(hi[a], l);
+                        }
+                    }
+                }
+                if (lo[0] == INT_MAX) continue;
+                long long volume = 1;
+                for (int a = 0; a < 3; ++a) volume *= std::min(hi[a] + 1, n[a] - 1) - lo[a] + 1;
+                ++slab_measured;
+                slab_over += volume > 2 * kBrickCapDefault;
+            }
             for (int q = 0; q < 3; ++q) {
                 const int k0 = sampled_chunks[q] * P.chunk_samples, k1 = std::min(k0 + P.chunk_samples, P.samples);
                 int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -352,7 +439,17 @@ int choose_brick_cap(const Params &P, const double *panels) {
                 for (int a = 0; a < 3; ++a) volume *= std::min(hi[a] + 1, n[a] - 1) - lo[a] + 1;
                 ++measured;
                 rescued += volume > kBrickCapDefault && volume <= 2 * kBrickCapDefault;
+                index_over += volume > 2 * kBrickCapDefault;
             }
+        }
+    // Slab chunks cost lockstep iterations (a wave walks each slab as long as
+    // its lane with the most samples there), so a plan takes them only when
+    // index chunks overflow even the larger brick in at least 1/16 of the
+    // samples and slab chunks in less than half as many: then with the
+    // larger brick.
+    *slab_chunks = measured > 0 && slab_measured > 0 && 16 * index_over >= measured &&
+                   2 * slab_over * measured < index_over * slab_measured;
+    if (*slab_chunks) return 2 * kBrickCapDefault;
     return measured > 0 && 4 * rescued >= measured ? 2 * kBrickCapDefault : kBrickCapDefault;
 }
 
@@ -805,7 +902,106 @@ __global__ __launch_bounds__(kBlock) void forward64_kernel(Params P, const T *__
 }
 
 // kernels/projection.py:873-1237 (midpoint, FP64 sampling, FP32 increments)
-template <typename T, bool VOL, bool POSE, int C>
+// Slab chunks of the FP64 volume adjoint.  Chunks that follow the sample
+// index put the k-th samples of a tile's rays in one brick, which spreads
+// apart where the tile's clipped intervals differ in length (at the grid's
+// silhouette) and overflows the brick.  Here the workgroup cuts space into
+// slabs of W grid samples along its dominant axis A (the axis most of its
+// rays advance fastest along) and each lane visits, per chunk, the samples
+// whose A coordinate lies in the chunk's slab; W is the advance of the
+// fastest lane over the plan's chunk length.  Slab boundaries are integers
+// and both sides of a boundary use the same expression, so every sample
+// falls in exactly one chunk, and a lane's chunks come in its sample order.
+struct Slabs {
+    int axis, lo, width, count;
+};
+
+__device__ __forceinline__ double pick3(const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+
+__device__ __forceinline__ Slabs slab_setup(const Params &P, bool active, const Ray &r, const Interval64 &iv,
+                                            double step, int (*red)[6]) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double ad[3] = {fabs(r.d[0]), fabs(r.d[1]), fabs(r.d[2])};
+    const int mine = ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2);
+    int votes[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) votes[a] = __popcll(__builtin_amdgcn_ballot_w64(active && mine == a));
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) red[wv][a] = votes[a];
+    __syncthreads();
+    int total[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) total[a] = red[0][a] + red[1][a] + red[2][a] + red[3][a];
+    const int axis = total[0] >= total[1] && total[0] >= total[2] ? 0 : (total[1] >= total[2] ? 1 : 2);
+    const int n = axis == 0 ? P.nx : (axis == 1 ? P.ny : P.nz);
+    int lo = INT_MAX, nhi = INT_MAX, nadv = INT_MAX;
+    if (active) {
+        const double oa = pick3(r.o, axis), da = pick3(r.d, axis);
+        const double q0 = oa + (iv.lower + 0.5 * step) * da;
+        const double q1 = oa + (iv.lower + ((double)P.samples - 0.5) * step) * da;
+        lo = (int)floor(fmin(fmax(fmin(q0, q1), -2.0), (double)n + 1.0));
+        nhi = -(int)ceil(fmin(fmax(fmax(q0, q1), -2.0), (double)n + 1.0));
+        // advance per sample in 1/64 grid samples, at least 1
+        nadv = -(int)fmin(fmax(ceil(step * fabs(da) * 64.0), 1.0), 1048576.0);
+    }
+    lo = wave_min(lo);
+    nhi = wave_min(nhi);
+    nadv = wave_min(nadv);
+    __syncthreads();  // every wave has read the votes
+    if (lane == 0) {
+        red[wv][3] = lo;
+        red[wv][4] = nhi;
+        red[wv][5] = nadv;
+    }
+    __syncthreads();
+    Slabs out;
+    out.axis = axis;
+    out.lo = min(min(red[0][3], red[1][3]), min(red[2][3], red[3][3]));
+    const int hi = -min(min(red[0][4], red[1][4]), min(red[2][4], red[3][4]));
+    const int adv = -min(min(red[0][5], red[1][5]), min(red[2][5], red[3][5]));
+    __syncthreads();  // red is reused by the chunk bounds
+    if (out.lo == INT_MAX) {
+        out.lo = 0;
+        out.width = 1;
+        out.count = 1;
+        return out;
+    }
+    out.width = max(1, (P.chunk_samples * adv + 63) / 64);
+    out.count = max(1, (hi - out.lo + out.width - 1) / out.width);
+    return out;
+}
+
+// Sample range [k0, k1) of this lane in chunk j.
+__device__ __forceinline__ void slab_range(const Params &P, bool active, const Ray &r, const Interval64 &iv,
+                                           double step, const Slabs &sl, int j, int &k0, int &k1) {
+    k0 = k1 = 0;
+    if (!active) return;
+    const double oa = pick3(r.o, sl.axis), da = pick3(r.d, sl.axis);
+    const double S = (double)P.samples;
+    if (da == 0.0) {
+        const int own = min(max((int)floor((oa - (double)sl.lo) / (double)sl.width), 0), sl.count - 1);
+        k1 = j == own ? P.samples : 0;
+        return;
+    }
+    // t(b) = ((b - o) / d - lower) / step - 0.5, monotone in b: the sample
+    // index where the coordinate crosses the boundary b
+    const int js = da > 0.0 ? j : sl.count - 1 - j;
+    const double b0 = (double)(sl.lo + js * sl.width), b1 = (double)(sl.lo + (js + 1) * sl.width);
+    const double t0 = ((b0 - oa) / da - iv.lower) / step - 0.5;
+    const double t1 = ((b1 - oa) / da - iv.lower) / step - 0.5;
+    if (da > 0.0) {
+        // coordinate(k) >= b  <=>  k >= t(b)
+        k0 = j == 0 ? 0 : (int)fmin(fmax(ceil(t0), 0.0), S);
+        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(ceil(t1), 0.0), S);
+    } else {
+        // coordinate(k) < b  <=>  k > t(b)
+        k0 = j == 0 ? 0 : (int)fmin(fmax(floor(t1) + 1.0, 0.0), S);
+        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(floor(t0) + 1.0, 0.0), S);
+    }
+    k1 = max(k1, k0);
+}
+
+template <typename T, bool VOL, bool POSE, int C, bool SLABS = false>
 __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__restrict__ field,
                                                        const double *__restrict__ seeds,
                                                        double *__restrict__ acc,
@@ -862,15 +1058,23 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
     }
     double v[8];
     int cur = -1;  // corners cached while consecutive samples share a cell
-    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
+    // SLABS (volume adjoint only, chosen per plan): chunks by slabs of space
+    Slabs slabs{0, 0, 1, 1};
+    if constexpr (VOL && SLABS) slabs = slab_setup(P, active, r, iv, step, bound_red);
+    const int chunks = (VOL && SLABS) ? slabs.count : (P.samples + P.chunk_samples - 1) / P.chunk_samples;
     for (int ch = 0; ch < chunks; ++ch) {
-        const int k0 = ch * P.chunk_samples;
-        const int k1 = min(k0 + P.chunk_samples, P.samples);
+        int k0, k1;
+        if constexpr (VOL && SLABS) {
+            slab_range(P, active, r, iv, step, slabs, ch, k0, k1);
+        } else {
+            k0 = ch * P.chunk_samples;
+            k1 = min(k0 + P.chunk_samples, P.samples);
+        }
         Brick B{};
         bool use_lds = false;
         if constexpr (VOL) {
             int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
-            if (active) {
+            if (active && k1 > k0) {
                 // sample positions are monotone in k: the chunk's end samples bound it
                 const double pa = iv.lower + ((double)k0 + 0.5) * step;
                 const double pb = iv.lower + ((double)(k1 - 1) + 0.5) * step;
@@ -1846,8 +2050,14 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
         const double *sd = static_cast<const double *>(seeds);
         const size_t lds = (size_t)P.brick_cap * sizeof(double);
         double *partials = pose_destination<C>(p);
-        if (vol && pose)
+        if (vol && pose && P.slab_chunks)
+            hipLaunchKernelGGL((vjp64_kernel<T, true, true, C, true>), grid, block, lds, s, P, f, sd, p->d_acc,
+                               partials);
+        else if (vol && pose)
             hipLaunchKernelGGL((vjp64_kernel<T, true, true, C>), grid, block, lds, s, P, f, sd, p->d_acc,
+                               partials);
+        else if (vol && P.slab_chunks)
+            hipLaunchKernelGGL((vjp64_kernel<T, true, false, C, true>), grid, block, lds, s, P, f, sd, p->d_acc,
                                partials);
         else if (vol)
             hipLaunchKernelGGL((vjp64_kernel<T, true, false, C>), grid, block, lds, s, P, f, sd, p->d_acc,
@@ -2026,7 +2236,8 @@ int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t 
         const int forced = std::atoi(chunk);
         if (forced >= 1 && forced <= 1024) P.chunk_samples = forced;
     }
-    P.brick_cap = P.bundle ? kBrickCapDefault : choose_brick_cap(P, panels);
+    P.slab_chunks = 0;
+    P.brick_cap = P.bundle ? kBrickCapDefault : choose_brick_cap(P, panels, &P.slab_chunks);
     if (const char *cap = std::getenv("DPT_AMD_BRICK_CAP")) {
         const int forced = std::atoi(cap);
         if (forced >= 512 && forced <= 8192) P.brick_cap = forced;
