@@ -102,7 +102,7 @@ def kernels_once(args) -> int:
     import numpy as np
     import torch
 
-    from dpt_amd import _native, phantoms
+    from dpt import _native, phantoms
 
     grid = phantoms.centred_grid(SHAPE)
     values = phantoms.chest_phantom(grid, seed=2026)
@@ -143,9 +143,9 @@ def serving_workload(count: int, seed: int = 404):
     import numpy as np
     import torch
 
-    from dpt_amd import phantoms, serving
-    from dpt_amd.geometry import ConeBeamIntrinsics, RigidTransform, compose_pose
-    from dpt_amd.torch import Projector
+    from dpt import phantoms, serving
+    from dpt.geometry import ConeBeamIntrinsics, RigidTransform, compose_pose
+    from dpt.torch import Projector
 
     rng = np.random.default_rng(seed)
     shapes = [((128, 128, 128), (1.0, 1.0, 1.0)), ((256, 256, 256), (1.0, 1.0, 1.0)), ((76, 507, 512), (0.45, 0.45, 2.5))]
@@ -188,7 +188,7 @@ def serving_benchmark(count: int, rate: float) -> dict:
 
     import numpy as np
 
-    from dpt_amd import serving
+    from dpt import serving
 
     volumes, requests, truths = serving_workload(count)
     arrivals = serving.poisson_arrivals(count, rate, seed=405)
@@ -246,7 +246,7 @@ def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
     tallies, moments, one status readback) and of the trace kernel alone."""
     import torch
 
-    from dpt_amd import transport
+    from dpt import transport
 
     out = {"workload": "128^3 chest labels (air/soft tissue/bone at 0.0012/1.0/1.85 g/cm^3), 1 mm, "
                        "SOD/SDD 1000/1500 mm, 256^2 detector, Klein-Nishina + Rayleigh, Woodcock majorant",
@@ -280,7 +280,7 @@ def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
         if cpu:
             from oracle import transport as T
 
-            from dpt_amd.transport.simulation import bind_model
+            from dpt.transport.simulation import bind_model
 
             model = T.Model(*bind_model(spec, material_ids=labels, absorption=a, scattering=s, physical=physical))
             t0 = time.perf_counter()
@@ -311,7 +311,7 @@ def precision_benchmark(grid, geometry, values, target, samples_list=(512, 1024)
     import numpy as np
     import torch
 
-    from dpt_amd.torch import Projector
+    from dpt.torch import Projector
 
     device = torch.device("cuda:0")
     field = torch.tensor(values, device=device, requires_grad=True)
@@ -378,8 +378,8 @@ def gather_benchmark(grid, geometry, values, target, samples: int) -> dict:
     and compared with the atomic FP64 volume VJP of the same cotangent."""
     import torch
 
-    from dpt_amd.projection import prepare
-    from dpt_amd.torch import Projector
+    from dpt.projection import prepare
+    from dpt.torch import Projector
 
     device = torch.device("cuda:0")
     module = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)
@@ -454,10 +454,10 @@ def gather_chest_benchmark(repeats: int = 10) -> dict:
     import numpy as np
     import torch
 
-    from dpt_amd.geometry import DetectorGeometry
-    from dpt_amd.projection import prepare
-    from dpt_amd.torch import Projector
-    from dpt_amd.volumes import GridSpec
+    from dpt.geometry import DetectorGeometry
+    from dpt.projection import prepare
+    from dpt.torch import Projector
+    from dpt.volumes import GridSpec
 
     device = torch.device("cuda:0")
     grid = GridSpec(**CHEST_GRID)
@@ -518,7 +518,7 @@ def cell_gauss_benchmark(grid, geometry, values, target, samples: int) -> dict:
     rule at ``samples`` against it (relative L2 of the images)."""
     import torch
 
-    from dpt_amd.torch import Projector
+    from dpt.torch import Projector
 
     device = torch.device("cuda:0")
     exact = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples,
@@ -573,7 +573,7 @@ def gpu_parity(grid, geometry, values, samples, picks, image, seed, grad_field, 
     import numpy as np
     import torch
 
-    from dpt_amd import _native
+    from dpt import _native
 
     panels = np.asarray([geometry.views[k].packed() for k in picks], dtype=np.float64)
     rows, cols = geometry.shape_rows_columns
@@ -620,8 +620,8 @@ def cpu_baseline(grid, geometry, values, samples: int, stride: int) -> dict:
 
     import numpy as np
 
-    from dpt_amd import phantoms
-    from dpt_amd.geometry import ConeBeamIntrinsics
+    from dpt import phantoms
+    from dpt.geometry import ConeBeamIntrinsics
     from oracle import oracle as O
 
     def run(picks, row_stride, scene_grid, scene_values, geom, s, seed_fn):
@@ -718,9 +718,9 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from dpt_amd import phantoms
-    from dpt_amd.projection import decode, gather_cells, prepare
-    from dpt_amd.torch import Projector
+    from dpt import phantoms
+    from dpt.projection import decode, gather_cells, prepare
+    from dpt.torch import Projector
 
     device = torch.device("cuda:0")
     _log("building phantom and geometry")

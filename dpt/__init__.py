@@ -1,4 +1,4 @@
-"""dpt_amd: MI355X-native differentiable cone-beam X-ray projector.
+"""dpt: MI355X-native differentiable cone-beam X-ray projector.
 
 Drop-in for the reference's ``dpt.project()`` / ``dpt.torch.Projector`` path:
 ``(views, rows, columns)`` optical depth with volume and 6-DoF pose VJPs, run

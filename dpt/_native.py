@@ -1,6 +1,6 @@
 """ctypes binding of the C ABI declared in include/dpt_amd.h.
 
-The shared library is built in-tree by ``python -m dpt_amd.build_native``.  The
+The shared library is built in-tree by ``python -m dpt.build_native``.  The
 product path has no fallback: if the library is missing, every projector call
 raises ``NativeLibraryError``.
 """
@@ -164,8 +164,8 @@ def lib():
             return _LIB
         if not LIB_PATH.exists():
             raise NativeLibraryError(
-                f"dpt_amd HIP library not found at {LIB_PATH}; build it with "
-                "`python -m dpt_amd.build_native` (no CPU fallback exists)"
+                f"dpt HIP library not found at {LIB_PATH}; build it with "
+                "`python -m dpt.build_native` (no CPU fallback exists)"
             )
         h = ctypes.CDLL(str(LIB_PATH))
         vp, i32, u32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64

@@ -54,7 +54,7 @@ def project(
     from .torch.modules import _geometry
 
     if prepared is not None and not isinstance(prepared, Projector):
-        raise ContractError("prepared must be a dpt_amd.torch.Projector")
+        raise ContractError("prepared must be a dpt.torch.Projector")
     if isinstance(volume, ImagingVolume):
         if volume.value_units != "mm^-1":
             raise ContractError("projection requires attenuation units='mm^-1'; convert HU first")

@@ -1,6 +1,6 @@
-"""Compile the gfx950 HIP library ``dpt_amd/_lib/libdpt_amd.so`` in-tree.
+"""Compile the gfx950 HIP library ``dpt/_lib/libdpt_amd.so`` in-tree.
 
-Run as ``python -m dpt_amd.build_native``.  hipcc cross-compiles without a
+Run as ``python -m dpt.build_native``.  hipcc cross-compiles without a
 GPU.  Flags:
 
 * ``-ffp-contract=off``: every fused multiply-add in the kernels is explicit,
@@ -52,7 +52,7 @@ FLAGS = [
 def hipcc() -> str:
     found = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(found).exists():
-        raise RuntimeError("hipcc not found; the ROCm toolchain is required to build dpt_amd")
+        raise RuntimeError("hipcc not found; the ROCm toolchain is required to build dpt")
     return found
 
 

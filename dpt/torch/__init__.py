@@ -1,4 +1,4 @@
-"""Torch adapter: ``Projector``, ``compose_pose`` and the ``dpt_amd::`` operators."""
+"""Torch adapter: ``Projector``, ``compose_pose`` and the ``dpt::`` operators."""
 
 from .geometry import compose_pose
 from .modules import Projector

@@ -1,12 +1,12 @@
 """Checked multiview Torch operators over the native HIP plan.
 
 Same operator contract as the reference (python/dpt/torch/_projection_ops.py:
-256-411), registered in the ``dpt_amd`` namespace:
+256-411), registered in the ``dpt`` namespace:
 
-* ``dpt_amd::project(field, poses, value, active_volume, active_pose) -> image``
-* ``dpt_amd::project_vjp(field, poses, cotangent, value, active_volume,
+* ``dpt::project(field, poses, value, active_volume, active_pose) -> image``
+* ``dpt::project_vjp(field, poses, cotangent, value, active_volume,
   active_pose) -> (grad_field | empty(0), grad_poses | empty(0))``
-* ``dpt_amd::project_out(field, poses, value, out!)``
+* ``dpt::project_out(field, poses, value, out!)``
 
 plus ``adjoint_gather``, the deterministic FP64 transpose (an explicit,
 non-autograd call).
@@ -45,7 +45,7 @@ def _metadata(field: torch.Tensor, poses: torch.Tensor, value: str):
 def _require_device(field: torch.Tensor) -> None:
     if field.device.type != "cuda":
         raise ContractError(
-            "dpt_amd executes on ROCm GPUs only; move field and poses to a cuda device"
+            "dpt executes on ROCm GPUs only; move field and poses to a cuda device"
         )
 
 
@@ -87,7 +87,7 @@ def _forward(field, poses, value, active_volume, active_pose, out=None):
     return images
 
 
-@torch.library.custom_op("dpt_amd::project", mutates_args=(), tags=_TAGS)
+@torch.library.custom_op("dpt::project", mutates_args=(), tags=_TAGS)
 def project(
     field: torch.Tensor, poses: torch.Tensor, value: str, active_volume: bool, active_pose: bool
 ) -> torch.Tensor:
@@ -100,7 +100,7 @@ def _project_fake(field, poses, value, active_volume, active_pose):
     return torch.empty(shape, dtype=dtype, device=field.device)
 
 
-@torch.library.custom_op("dpt_amd::project_vjp", mutates_args=(), tags=_TAGS)
+@torch.library.custom_op("dpt::project_vjp", mutates_args=(), tags=_TAGS)
 def project_vjp(
     field: torch.Tensor,
     poses: torch.Tensor,
@@ -253,7 +253,7 @@ def adjoint_gather(poses: torch.Tensor, cotangent: torch.Tensor, value: str, out
     return out
 
 
-@torch.library.custom_op("dpt_amd::project_out", mutates_args={"out"}, tags=_TAGS)
+@torch.library.custom_op("dpt::project_out", mutates_args={"out"}, tags=_TAGS)
 def project_out(field: torch.Tensor, poses: torch.Tensor, value: str, out: torch.Tensor) -> None:
     _forward(field, poses, value, False, False, out)
 

@@ -1,5 +1,5 @@
 /*
- * dpt_amd -- C ABI of the MI355X cone-beam projector.
+ * dpt -- C ABI of the MI355X cone-beam projector.
  *
  * Plain pointers and sizes only: device pointers come from the caller's
  * allocator (PyTorch, hipMalloc, ...), streams are hipStream_t passed as

@@ -3,7 +3,7 @@
  *
  * This file is the oracle: only tests, the smoke check and the CPU-baseline
  * leg of bench.py may load it, and only as the checker.  The product path
- * (dpt_amd) never calls into it.
+ * (dpt) never calls into it.
  *
  * It restates, operation for operation, two reference kernel families:
  *

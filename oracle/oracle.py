@@ -2,7 +2,7 @@
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this module, and only as the checker.  The product
-package ``dpt_amd`` never imports it.
+package ``dpt`` never imports it.
 
 Two layers:
 

@@ -3,7 +3,7 @@
  *
  * This file is part of the oracle: only tests, the smoke check and the
  * CPU-baseline leg of bench.py may load it, and only as the checker.  The
- * product path (dpt_amd) never calls into it.
+ * product path (dpt) never calls into it.
  *
  * It restates, in binary64 and operation for operation, the analogue
  * Woodcock transport of the reference:

@@ -25,8 +25,8 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import _native  # noqa: E402
-from dpt_amd.contracts import ContractError  # noqa: E402
+from dpt import _native  # noqa: E402
+from dpt.contracts import ContractError  # noqa: E402
 
 from .test_gpu_gather import _case  # noqa: E402
 from .test_gpu_parity import NativeScene  # noqa: E402
@@ -79,8 +79,8 @@ def test_projector_cell_gauss_is_exact_and_differentiable():
     """Projector(precision="fp64", integration="cell_gauss"): a constant field
     integrates to value x clipped length at any sample count; the midpoint
     rule converges to the cell-Gauss image; autograd gives the adjoint."""
-    from dpt_amd import phantoms
-    from dpt_amd.torch import Projector
+    from dpt import phantoms
+    from dpt.torch import Projector
 
     grid = phantoms.centred_grid((12, 13, 14))
     geometry, _ = phantoms.circular_geometry(grid, 3, (10, 11), sod=60.0, sdd=100.0)

@@ -22,7 +22,7 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import _native  # noqa: E402
+from dpt import _native  # noqa: E402
 
 from .test_gpu_parity import NativeScene, _cone  # noqa: E402
 

@@ -31,8 +31,8 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import _native  # noqa: E402
-from dpt_amd.contracts import ContractError  # noqa: E402
+from dpt import _native  # noqa: E402
+from dpt.contracts import ContractError  # noqa: E402
 
 from .test_gpu_parity import NativeScene, _cone  # noqa: E402
 
@@ -181,8 +181,8 @@ def test_gather_matches_adjoint_identity_and_scatter_at_size():
 def test_projector_adjoint_gather_matches_the_oracle_and_the_module_adjoint():
     """Projector.adjoint_gather: FP64 and FP32 destinations, accumulation,
     and the contract errors of projection_adjoint_gather."""
-    from dpt_amd import phantoms
-    from dpt_amd.torch import Projector
+    from dpt import phantoms
+    from dpt.torch import Projector
 
     grid = phantoms.centred_grid((14, 15, 16))
     geometry, _ = phantoms.circular_geometry(grid, 3, (12, 13), sod=60.0, sdd=100.0)

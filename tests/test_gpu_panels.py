@@ -23,9 +23,9 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd.geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry  # noqa: E402
-from dpt_amd.torch import Projector  # noqa: E402
-from dpt_amd.volumes import GridSpec  # noqa: E402
+from dpt.geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry  # noqa: E402
+from dpt.torch import Projector  # noqa: E402
+from dpt.volumes import GridSpec  # noqa: E402
 
 
 def _setup(samples=96):

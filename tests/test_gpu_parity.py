@@ -3,7 +3,7 @@
 Bars: FP32 images, volume and pose gradients within 1e-5 relative L2 of the
 oracle's restatement of the reference FP32 path (per-ray quantities are
 compared much tighter); FP64 within 1e-10.  Every call goes through the C ABI
-(``dpt_amd._native.Plan``) or the ``Projector`` built on it.
+(``dpt._native.Plan``) or the ``Projector`` built on it.
 """
 
 from __future__ import annotations
@@ -20,8 +20,8 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import _native  # noqa: E402
-from dpt_amd.volumes import GridSpec  # noqa: E402
+from dpt import _native  # noqa: E402
+from dpt.volumes import GridSpec  # noqa: E402
 
 
 def _grid_spec(grid: O.Grid) -> GridSpec:
@@ -249,7 +249,7 @@ def test_closed_form_unit_voxel_through_the_c_abi(precision):
 
 def test_config1_forward_and_volume_vjp_match_oracle():
     """64^3 phantom, AP + lateral, 128^2 at 0.8 mm, 256 samples, central-64 mean loss."""
-    from dpt_amd import phantoms
+    from dpt import phantoms
 
     grid_spec = phantoms.centred_grid((64, 64, 64))
     values = phantoms.chest_phantom(grid_spec, seed=1)

@@ -25,10 +25,10 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import _native  # noqa: E402
-from dpt_amd.geometry import ConeBeamIntrinsics  # noqa: E402
-from dpt_amd.torch import Projector  # noqa: E402
-from dpt_amd.volumes import GridSpec  # noqa: E402
+from dpt import _native  # noqa: E402
+from dpt.geometry import ConeBeamIntrinsics  # noqa: E402
+from dpt.torch import Projector  # noqa: E402
+from dpt.volumes import GridSpec  # noqa: E402
 
 from .test_gpu_gather import _case  # noqa: E402
 from .test_gpu_parity import NativeScene  # noqa: E402
@@ -113,7 +113,7 @@ def test_cone_intrinsics_vjp_matches_central_differences():
 def test_projector_ray_vjp_matches_the_oracle(mode):
     """Projector.ray_vjp through the host layer (a plan made without pose
     buffers), against the oracle's ray-coordinate mode."""
-    from dpt_amd.geometry import DetectorGeometry
+    from dpt.geometry import DetectorGeometry
 
     precision, integration = mode
     grid = O.Grid((10, 11, 12), (1.0, 1.0, 1.0), (-5.5, -5.0, -4.5))
@@ -136,7 +136,7 @@ def test_ray_vjp_with_half_storage_matches_the_oracle_on_the_rounded_field(stora
     """Half storage rounds the field once; the ray derivatives of that field
     (FP32 sampling, compensated sums) match the oracle's FP32 path on the
     rounded values."""
-    from dpt_amd.geometry import DetectorGeometry
+    from dpt.geometry import DetectorGeometry
 
     grid = O.Grid((10, 11, 12), (1.0, 1.0, 1.0), (-5.5, -5.0, -4.5))
     panels = scenes.cone_panels(grid, 2, 8, 9, sod=50.0, sdd=90.0, offset=0.2)

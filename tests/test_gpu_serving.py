@@ -12,9 +12,9 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import phantoms, serving  # noqa: E402
-from dpt_amd.geometry import ConeBeamIntrinsics, RigidTransform, compose_pose  # noqa: E402
-from dpt_amd.torch import Projector  # noqa: E402
+from dpt import phantoms, serving  # noqa: E402
+from dpt.geometry import ConeBeamIntrinsics, RigidTransform, compose_pose  # noqa: E402
+from dpt.torch import Projector  # noqa: E402
 
 
 def _scene(n=40, views=3, det=40, seed=0, angles=None):

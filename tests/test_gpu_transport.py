@@ -21,7 +21,7 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import transport  # noqa: E402
+from dpt import transport  # noqa: E402
 from oracle import transport as T  # noqa: E402
 
 from . import transport_scenes as S  # noqa: E402
@@ -189,7 +189,7 @@ def test_guards_raise_typed_errors():
 
 
 def test_object_pose_moves_the_phantom_not_the_detector():
-    from dpt_amd.geometry import RigidTransform
+    from dpt.geometry import RigidTransform
 
     scene = S.homogeneous(0.03, 0.0, pose=RigidTransform(translation_mm=(0.0, 0.0, 20.0)))
     ws = _workspace(scene, 20000)

@@ -18,9 +18,9 @@ torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-from dpt_amd import ContractError, NumericalError, TrialDomainError, phantoms  # noqa: E402
-from dpt_amd.geometry import RigidTransform  # noqa: E402
-from dpt_amd.torch import Projector  # noqa: E402
+from dpt import ContractError, NumericalError, TrialDomainError, phantoms  # noqa: E402
+from dpt.geometry import RigidTransform  # noqa: E402
+from dpt.torch import Projector  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
@@ -178,10 +178,10 @@ def test_torch_compile_fullgraph_traces_the_custom_ops():
 
 
 def test_api_project_matches_the_module():
-    import dpt_amd
+    import dpt
 
     grid, values, geometry, samples = _scene(n=16, views=2, det=16, samples=40)
-    volume = dpt_amd.ImagingVolume(np.asarray(values), grid)
-    image = dpt_amd.project(volume, geometry, device="cuda:0", samples_per_ray=samples)
+    volume = dpt.ImagingVolume(np.asarray(values), grid)
+    image = dpt.project(volume, geometry, device="cuda:0", samples_per_ray=samples)
     module = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)
     torch.testing.assert_close(image, module(torch.tensor(values, device="cuda")), rtol=0, atol=0)

@@ -12,14 +12,14 @@ import numpy as np
 import pytest
 import torch
 
-import dpt_amd
-from dpt_amd import _native, phantoms
-from dpt_amd.contracts import ContractError, NumericalError, TrialDomainError, raise_view_statuses, status_error
-from dpt_amd.geometry import BatchedGeometry, ConeBeamIntrinsics, DetectorGeometry, RigidTransform, compose_pose
-from dpt_amd.projection import ProjectionSpec, configuration, decode
-from dpt_amd.torch import Projector
-from dpt_amd.torch import compose_pose as torch_compose
-from dpt_amd.volumes import GridSpec
+import dpt
+from dpt import _native, phantoms
+from dpt.contracts import ContractError, NumericalError, TrialDomainError, raise_view_statuses, status_error
+from dpt.geometry import BatchedGeometry, ConeBeamIntrinsics, DetectorGeometry, RigidTransform, compose_pose
+from dpt.projection import ProjectionSpec, configuration, decode
+from dpt.torch import Projector
+from dpt.torch import compose_pose as torch_compose
+from dpt.volumes import GridSpec
 from tests import scenes
 
 HEADER = Path(__file__).resolve().parent.parent / "include" / "dpt_amd.h"
@@ -206,13 +206,13 @@ def test_fake_tensor_tracing_gives_output_and_gradient_metadata():
 
 
 def test_custom_op_schemas_match_the_reference_operators():
-    from dpt_amd.torch import _projection_ops  # noqa: F401
+    from dpt.torch import _projection_ops  # noqa: F401
 
-    project = torch.ops.dpt_amd.project.default._schema
+    project = torch.ops.dpt.project.default._schema
     assert [a.name for a in project.arguments] == ["field", "poses", "value", "active_volume", "active_pose"]
-    vjp = torch.ops.dpt_amd.project_vjp.default._schema
+    vjp = torch.ops.dpt.project_vjp.default._schema
     assert [a.name for a in vjp.arguments] == ["field", "poses", "cotangent", "value", "active_volume", "active_pose"]
-    out = torch.ops.dpt_amd.project_out.default._schema
+    out = torch.ops.dpt.project_out.default._schema
     assert [a.name for a in out.arguments if a.alias_info and a.alias_info.is_write] == ["out"]
 
 
@@ -262,11 +262,11 @@ def test_coverage_pitch_covers_every_support_corner():
 def test_api_project_argument_rules():
     grid = phantoms.centred_grid((4, 4, 4))
     with pytest.raises(ContractError, match="prepared"):
-        dpt_amd.project(torch.zeros(grid.shape))
+        dpt.project(torch.zeros(grid.shape))
     with pytest.raises(ContractError, match="geometry is required"):
-        dpt_amd.project(dpt_amd.ImagingVolume(np.zeros(grid.shape), grid), device="meta")
+        dpt.project(dpt.ImagingVolume(np.zeros(grid.shape), grid), device="meta")
     with pytest.raises(ContractError, match="ImagingVolume"):
-        dpt_amd.project(np.zeros(grid.shape))
+        dpt.project(np.zeros(grid.shape))
 
 
 def test_batched_geometry_requires_equal_shapes():
@@ -281,9 +281,9 @@ def test_parallel_and_cylindrical_panels_validate_and_round_trip():
     view's kind."""
     import math
 
-    from dpt_amd.geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry
-    from dpt_amd.projection import ProjectionSpec, configuration, decode
-    from dpt_amd.volumes import GridSpec
+    from dpt.geometry import BatchedGeometry, CurvedDetectorGeometry, DetectorGeometry, ParallelGeometry
+    from dpt.projection import ProjectionSpec, configuration, decode
+    from dpt.volumes import GridSpec
 
     parallel = ParallelGeometry((5.0, 0.0, 0.0), (1.0, 0.0, 0.0), 10.0, (0.0, 1.0, 0.0), (0.0, 0.0, 1.0),
                                 (0.5, 0.5), (3, 4))

@@ -147,7 +147,7 @@ def test_klein_nishina_cosines_and_energies():
 
 
 def test_rayleigh_cosines_follow_form_factor_law():
-    from dpt_amd.transport import materials
+    from dpt.transport import materials
 
     spec, physical, labels, a, s, dens, source = S.chest(shape=(8, 8, 8), detector=(8, 8))
     model = _model((spec, physical, labels, a, s))
@@ -213,7 +213,7 @@ def test_history_results_depend_only_on_identity():
 
 def test_spectrum_lines_follow_their_cdf():
     spec, _, labels, a, s, source = S.homogeneous(0.0, 0.0)
-    from dpt_amd.transport import PhysicalTransport, materials
+    from dpt.transport import PhysicalTransport, materials
 
     physical = PhysicalTransport((materials.scattering_table(7),), (0.0, 0.0), spectrum=S.spectrum_two_lines())
     model = _model((spec, physical, labels, a, s))

@@ -6,8 +6,8 @@ import math
 
 import numpy as np
 
-from dpt_amd.geometry import DetectorGeometry, RigidTransform
-from dpt_amd.transport import (
+from dpt.geometry import DetectorGeometry, RigidTransform
+from dpt.transport import (
     AtomicScatteringTable,
     MaterialGrid,
     OrientedDetector,
@@ -16,8 +16,8 @@ from dpt_amd.transport import (
     SourceSpectrum,
     TransportSpec,
 )
-from dpt_amd.transport import materials
-from dpt_amd.transport.simulation import bind_model
+from dpt.transport import materials
+from dpt.transport.simulation import bind_model
 
 NAMESPACE = 0x80000000
 
