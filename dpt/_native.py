@@ -27,6 +27,7 @@ EXPORTS = (
     "dpt_ray_vjp",
     "dpt_reduce_rays",
     "dpt_adjoint_gather",
+    "dpt_adjoint_gather_noalloc",
     "dpt_status",
     "dpt_convert_field",
     "dpt_cell_count",
@@ -57,6 +58,8 @@ VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
 GATHER_CELLS = 64
 # plan option (needs, FP64 only): exact cell-Gauss integration
 CELL_GAUSS = 128
+# bytes of the fixed device workspace of dpt_adjoint_gather_noalloc
+GATHER_NOALLOC_BYTES = 32 << 20
 
 STORAGE_CODES = {"float32": STORE_F32, "bfloat16": STORE_BF16, "float16": STORE_F16}
 # 16-bit field buffers carry DPT_HALF_PAD_BYTES (8) of readable padding
@@ -190,6 +193,8 @@ def lib():
         h.dpt_reduce_rays.restype = ctypes.c_int
         h.dpt_adjoint_gather.argtypes = [vp, vp, vp, vp, i32, u32, vp]
         h.dpt_adjoint_gather.restype = ctypes.c_int
+        h.dpt_adjoint_gather_noalloc.argtypes = [vp, vp, vp, vp, i32, u32, vp]
+        h.dpt_adjoint_gather_noalloc.restype = ctypes.c_int
         h.dpt_status.argtypes = [vp, vp, i32, vp]
         h.dpt_status.restype = ctypes.c_int
         h.dpt_convert_field.argtypes = [vp, vp, i64, i32, vp, vp]
@@ -323,6 +328,16 @@ class Plan:
         check(
             lib().dpt_adjoint_gather(self.handle, poses, cotangent, out, ARITH_CODES[out_precision], flags, stream),
             "dpt_adjoint_gather",
+        )
+
+    def adjoint_gather_noalloc(self, poses: int, cotangent: int, out: int, out_precision: str, flags: int,
+                               stream: int) -> None:
+        """dpt_adjoint_gather on the library's fixed per-device workspace
+        (allocates nothing; calls overlapping on one GPU race)."""
+        check(
+            lib().dpt_adjoint_gather_noalloc(self.handle, poses, cotangent, out, ARITH_CODES[out_precision], flags,
+                                             stream),
+            "dpt_adjoint_gather_noalloc",
         )
 
     def status(self, stream: int, clear: bool = True) -> np.ndarray:

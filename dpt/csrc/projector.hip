@@ -35,6 +35,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -2592,6 +2593,81 @@ int dpt_adjoint_gather(dpt_plan *p, const double *poses, const double *cotangent
                           out_precision == DPT_ARITH_FP64, (flags & DPT_ACCUMULATE) != 0,
                           (flags & DPT_NO_CHECKS) == 0, p->d_panel_meta, p->d_panel_modes, p->image_cols, s));
     timer_end(p, 2, s);
+    return DPT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free gather --
+// Workspace of dpt_adjoint_gather_noalloc: the gather's ray table, view
+// table and guard word in one file-scope device array of fixed size.  Every
+// GPU the library runs on holds its own copy until the process ends; in host
+// code `scratch` names a host-side stand-in, so kernels and copies only ever
+// get the per-device address that hipGetSymbolAddress returns.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)DPT_GATHER_NOALLOC_BYTES;
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_lock;
+unsigned char *g_scratch[kScratchDevices] = {};
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// The device address of this device's scratch (looked up once per device,
+// with that device current).
+int scratch_address(int device, unsigned char **out) {
+    if (device < 0 || device >= kScratchDevices) return fail(DPT_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (g_scratch[device] == nullptr) {
+        DeviceGuard guard(device);
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) return fail(DPT_E_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch[device];
+    return DPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpt_adjoint_gather_noalloc(dpt_plan *p, const double *poses, const double *cotangent, void *out,
+                               int32_t out_precision, uint32_t flags, void *stream) {
+    if (p == nullptr || poses == nullptr || cotangent == nullptr || out == nullptr)
+        return fail(DPT_E_INVALID, "plan, poses, cotangent and out are required");
+    if (p->arithmetic != DPT_ARITH_FP64 || p->cell_gauss)
+        return fail(DPT_E_INVALID, "deterministic gather requires strict float64 midpoint sampling");
+    if (out_precision != DPT_ARITH_FP32 && out_precision != DPT_ARITH_FP64)
+        return fail(DPT_E_INVALID, "out_precision must be DPT_ARITH_FP32 or DPT_ARITH_FP64");
+    if (flags & ~(uint32_t)(DPT_ACCUMULATE | DPT_NO_CHECKS))
+        return fail(DPT_E_INVALID, "gather flags are DPT_ACCUMULATE and DPT_NO_CHECKS");
+    // [ray table | view table | guard], each 256-byte aligned
+    const size_t ray_bytes = align256(sizeof(GatherRay) * (size_t)p->pixels);
+    const size_t view_bytes = align256(sizeof(GatherView) * (size_t)p->base.views);
+    const size_t need = ray_bytes + view_bytes + 256;
+    if (need > kScratchBytes)
+        return fail(DPT_E_INVALID, "the allocation-free gather needs " + std::to_string(need) +
+                                       " bytes of workspace for " + std::to_string(p->pixels) + " rays and " +
+                                       std::to_string(p->base.views) + " views; its fixed workspace holds " +
+                                       std::to_string(kScratchBytes) + " (use dpt_adjoint_gather)");
+    unsigned char *base = nullptr;
+    const int rc = scratch_address(p->device, &base);
+    if (rc != DPT_OK) return rc;
+    DeviceGuard guard(p->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    GatherRay *rays = reinterpret_cast<GatherRay *>(base);
+    GatherView *views = reinterpret_cast<GatherView *>(base + ray_bytes);
+    int *guard_word = reinterpret_cast<int *>(base + ray_bytes + view_bytes);
+    // launch_gather clears the guard word with a kernel and rewrites both
+    // tables before the voxel pass reads them
+    HIP_TRY(launch_gather(launch_params(p, poses), cotangent, rays, views, guard_word, out,
+                          out_precision == DPT_ARITH_FP64, (flags & DPT_ACCUMULATE) != 0,
+                          (flags & DPT_NO_CHECKS) == 0, p->d_panel_meta, p->d_panel_modes, p->image_cols, s));
     return DPT_OK;
 }
 

@@ -204,6 +204,25 @@ int dpt_adjoint_gather(dpt_plan *plan, const double *poses, const double *cotang
                        int32_t out_precision, uint32_t flags, void *stream);
 
 /*
+ * dpt_adjoint_gather without any allocation: the same kernels and the same
+ * result bit for bit, with the ray table, view table and guard word in a
+ * fixed device workspace of DPT_GATHER_NOALLOC_BYTES that the library holds
+ * on every GPU it runs on until the process ends (no hipMalloc, hipFree or
+ * allocating library call, at the first call or later).  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The plan needs 112 bytes per ray plus 256-byte-aligned view and guard
+ * records within the workspace (one 512 x 512 view fits; larger plans get
+ * DPT_E_INVALID and should use dpt_adjoint_gather).  All calls on one GPU
+ * share the workspace: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller.  Arguments, flags and
+ * status behaviour as for dpt_adjoint_gather.
+ */
+#define DPT_GATHER_NOALLOC_BYTES (32u << 20)
+int dpt_adjoint_gather_noalloc(dpt_plan *plan, const double *poses, const double *cotangent, void *out,
+                               int32_t out_precision, uint32_t flags, void *stream);
+
+/*
  * Copy the per-view status words to host_codes (views int32) with one D2H
  * transfer and one stream synchronisation; clear them when `clear` != 0.
  * Replaces StatusCheckpoint.check/consume (python/dpt/_runtime.py:317-445).
