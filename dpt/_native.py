@@ -25,6 +25,7 @@ EXPORTS = (
     "dpt_forward",
     "dpt_vjp",
     "dpt_ray_vjp",
+    "dpt_pose_sensitivities",
     "dpt_reduce_rays",
     "dpt_adjoint_gather",
     "dpt_adjoint_gather_noalloc",
@@ -53,7 +54,7 @@ ARITH_FP32, ARITH_FP64 = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 # OR-ed into a storage code: the field is in the cell layout (dpt_convert_cells)
 STORE_CELLS = 8
-VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS = 1, 2, 4, 8, 16
+VOLUME, POSE, POSE_TANGENT, ACCUMULATE, NO_CHECKS, VOLUME_FP64 = 1, 2, 4, 8, 16, 32
 # plan option (needs): FP32 sampling gathers from a cell-layout copy
 GATHER_CELLS = 64
 # plan option (needs, FP64 only): exact cell-Gauss integration
@@ -188,6 +189,8 @@ def lib():
         h.dpt_vjp.argtypes = [vp, vp, vp, vp, vp, vp, u32, vp]
         h.dpt_vjp.restype = ctypes.c_int
         h.dpt_ray_vjp.argtypes = [vp, vp, vp, vp, vp, u32, vp]
+        h.dpt_pose_sensitivities.argtypes = [vp, vp, vp, vp, vp, u32, vp]
+        h.dpt_pose_sensitivities.restype = ctypes.c_int
         h.dpt_ray_vjp.restype = ctypes.c_int
         h.dpt_reduce_rays.argtypes = [vp, vp, vp, i32, vp, u32, vp]
         h.dpt_reduce_rays.restype = ctypes.c_int
@@ -319,6 +322,10 @@ class Plan:
 
     def ray_vjp(self, field: int, poses: int, cotangent: int, out: int, flags: int, stream: int) -> None:
         check(lib().dpt_ray_vjp(self.handle, field, poses, cotangent, out, flags, stream), "dpt_ray_vjp")
+
+    def pose_sensitivities(self, field: int, poses: int, cotangent: int, out: int, flags: int, stream: int) -> None:
+        check(lib().dpt_pose_sensitivities(self.handle, field, poses, cotangent, out, flags, stream),
+              "dpt_pose_sensitivities")
 
     def reduce_rays(self, rays: int, jacobians: int, components: int, out: int, flags: int, stream: int) -> None:
         check(lib().dpt_reduce_rays(self.handle, rays, jacobians, components, out, flags, stream), "dpt_reduce_rays")

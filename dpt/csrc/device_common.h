@@ -23,6 +23,9 @@ constexpr int kMaxComponents = 12;
 // Component count that selects per-ray source/endpoint derivatives (6 values
 // per ray, stored per ray instead of reduced) in the pose epilogue.
 constexpr int kRayCoordinates = 7;
+// Component count that selects the per-ray 6-DoF local-right tangent (tx, ty,
+// tz, rx, ry, rz per ray, stored per ray instead of reduced).
+constexpr int kRayTangent = 8;
 
 struct Params {
     int nx, ny, nz;
@@ -48,6 +51,7 @@ struct Params {
     int slab_chunks;         // 1: the FP64 volume adjoint chunks by slabs of space, not by sample index
     int chunk_samples;       // samples per lockstep chunk of the volume adjoint
     const float *alpha;      // 2 x samples entries: alpha_k, then k + 0.5 (sample_table_entries)
+    int volume_f64;          // 1: FP64 volume destination (increments unflushed, FP64 sampling unrounded)
 };
 
 // Per-sample constants, tabulated once on the host so that the sample loop
@@ -559,6 +563,13 @@ __device__ __forceinline__ void slopes32(const Cell32 &c, const float *v, float 
 }
 
 // kernels/projection_fp32.py:240-252: RN(factor*weight), |x| < 2^-126 -> signed 0
+// One FP64-sampled corner contribution as the volume destination receives
+// it: rounded once to FP32 for an FP32 destination, unrounded for an FP64
+// one (kernels/projection.py:407-430, get_scatter_realised).
+__device__ __forceinline__ double destination_increment(const Params &P, double contribution) {
+    return P.volume_f64 ? contribution : (double)(float)contribution;
+}
+
 __device__ __forceinline__ float scatter_increment(float factor, float weight) {
     const float x = factor * weight;
     return fabsf(x) < 0x1p-126f ? copysignf(0.0f, x) : x;

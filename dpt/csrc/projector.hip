@@ -18,11 +18,15 @@
 // gathered from the linear field (x-adjacent pairs) or from a cell-layout copy
 // (8 corners per cell, one or two 16-byte loads).  Pose cotangents are
 // reduced per ray in FP64, across the wave with shuffles, then across tiles in
-// a fixed-order FP64 pass.  Volume cotangents are summed per cell in
-// registers (FP32 increments, reference flush-to-zero rule), added per chunk
-// of samples into an FP64 LDS brick of the cells the workgroup touches, and
-// drained into an FP64 accumulator with one device atomic per touched grid
-// sample; a final pass rounds to FP32.
+// a fixed-order FP64 pass.  Volume cotangents: a ray whose factor
+// f = seed * length * step is finite with |f| >= 2^-102 sums its corner
+// weights per cell in FP32 registers and scales each sum by f in FP64 when the
+// cell is flushed into an FP64 LDS brick of the cells the workgroup touches
+// (the brick is drained into an FP64 accumulator with one device atomic per
+// touched grid sample); other rays add the reference's per-increment
+// RN(f * w), flushed to zero below 2^-126, with device atomics.  A final pass
+// rounds the accumulator (plus the previous gradient when accumulating) once
+// to FP32, or keeps FP64 for an FP64 destination.
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -131,12 +135,13 @@ __device__ __forceinline__ void wave_store(const double *g, double *partial, boo
     }
 }
 
-// Store the epilogue's result: per ray for ray coordinates (6 values at
-// partials + 6 * ray, non-finite flagged), else the wave tile's sum.
+// Store the epilogue's result: per ray for ray coordinates and per-ray
+// tangents (6 values at partials + 6 * ray, non-finite flagged), else the
+// wave tile's sum.
 template <int C>
 __device__ __forceinline__ void store_gradient(const Params &P, const double *g, double *partials, int wslot,
                                                bool wave_valid, int view, int row, int col) {
-    if constexpr (C == kRayCoordinates) {
+    if constexpr (C == kRayCoordinates || C == kRayTangent) {
         if (!ray_in_range(P, view, row, col)) return;
         const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
         bool bad = false;
@@ -561,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     const double *pose = P.poses + 12 * (size_t)view;
 
     // ---- per-ray setup
-    bool active = false, bounded = false;
+    bool active = false, weight_sums = false;
     float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, len = 0.0f, seed = 0.0f, span = 0.0f, step = 0.0f;
     float volume_factor = 0.0f;
     Interval32 iv{};
@@ -590,7 +595,16 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 active = true;
                 step = span / (float)P.samples;
                 volume_factor = seed * (len * step);
-                bounded = isfinite(volume_factor) && fabsf(volume_factor) <= 1.0f;
+                // Weight-sum form (cell sums of corner weights, scaled by the
+                // factor at the flush) for finite factors with |f| >= 2^-102.
+                // The reference rounds each increment RN(f * w) and flushes
+                // |RN(f * w)| < 2^-126 to a signed zero
+                // (kernels/projection_fp32.py:240-252); with |f| >= 2^-102
+                // that flush can only drop terms with w < 2^-24, each below
+                // 2^-126 in magnitude.  Smaller factors (where whole rays
+                // flush to exact zeros) and non-finite ones take the
+                // reference's per-increment form below.
+                weight_sums = isfinite(volume_factor) && fabsf(volume_factor) >= 0x1p-102f;
             }
         }
     }
@@ -622,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         if constexpr (VOL) {
             // ---- bound the cells of this chunk over the workgroup
             int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
-            if (active && bounded) {
+            if (active && weight_sums) {
                 const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
                 const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
                 int la[3], lb[3];
@@ -673,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 if (c.base != cur) {
                     if constexpr (POSE) load8<T, LAYOUT>(P, field, c.base, v);
                     if constexpr (VOL) {
-                        if (bounded) {
+                        if (weight_sums) {
                             if (use_lds)
                                 transition<true>(P, B, cs, c, brick, acc);
                             else
@@ -702,7 +716,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     shu.fma_add(alpha, along);
                 }
                 if constexpr (VOL) {
-                    if (bounded) {
+                    if (weight_sums) {
                         // The cell keeps FP32 sums of the corner weights; the
                         // ray's factor multiplies each sum once, in FP64, when
                         // the cell is flushed (instead of rounding seed * w
@@ -714,7 +728,11 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     } else {
 #pragma unroll
                         for (int kk = 0; kk < 8; ++kk) {
-                            const float inc = scatter_increment(volume_factor, wt[kk]);
+                            // FP32 destinations: the reference's flushed
+                            // increment; FP64 ones: RN(f * w) unflushed
+                            // (kernels/projection_fp32.py:212-235)
+                            const float inc = P.volume_f64 ? volume_factor * wt[kk]
+                                                           : scatter_increment(volume_factor, wt[kk]);
                             if (!isfinite(inc)) bad = true;
                             // c.base indexes cells in the cell layout: address the voxel
                             const int voxel = (c.lz * P.ny + c.ly) * P.nx + c.lx;
@@ -1126,14 +1144,14 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                         contribution *= (kk & 1) ? c.fx : 1.0 - c.fx;
                         contribution *= (kk & 2) ? c.fy : 1.0 - c.fy;
                         contribution *= (kk & 4) ? c.fz : 1.0 - c.fz;
-                        const float rounded = (float)contribution;
+                        const double rounded = destination_increment(P, contribution);
                         if (!isfinite(rounded)) bad = true;
-                        if (rounded == 0.0f) continue;
+                        if (rounded == 0.0) continue;
                         if (use_lds) {
                             const int off = ((kk & 1) ? B.lsx : 0) + ((kk & 2) ? B.lsy : 0) + ((kk & 4) ? B.lsz : 0);
-                            atomicAdd(&brick[local + off], (double)rounded);
+                            atomicAdd(&brick[local + off], rounded);
                         } else {
-                            unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], (double)rounded);
+                            unsafeAtomicAdd(&acc[c.base + corner_offset(P, kk)], rounded);
                         }
                     }
                 }
@@ -1398,11 +1416,11 @@ __global__ __launch_bounds__(kBlock) void forward_gauss_kernel(Params P, const T
     if (!isfinite(integral)) flag(P, view, DPT_STATUS_RANGE);
 }
 
-// FP32-rounded increment of one corner into the FP64 accumulator (zeros add
-// nothing and are skipped)
-__device__ __forceinline__ bool add_increment(double *acc, int index, double contribution) {
-    const float rounded = (float)contribution;
-    if (rounded != 0.0f) unsafeAtomicAdd(acc + index, (double)rounded);
+// Increment of one corner into the FP64 accumulator, rounded as the
+// destination requires (zeros add nothing and are skipped)
+__device__ __forceinline__ bool add_increment(const Params &P, double *acc, int index, double contribution) {
+    const double rounded = destination_increment(P, contribution);
+    if (rounded != 0.0) unsafeAtomicAdd(acc + index, rounded);
     return isfinite(rounded);
 }
 
@@ -1567,11 +1585,11 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
 #pragma unroll
                                 for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
                                 if (carried) {
-                                    const float rounded = (float)(first + second);
+                                    const double rounded = destination_increment(P, first + second);
                                     bad |= !isfinite(rounded);
-                                    carry.v[k] += (double)rounded;
+                                    carry.v[k] += rounded;
                                 } else {
-                                    bad |= !add_increment(acc, corner_of(P, s.lower, k), first + second);
+                                    bad |= !add_increment(P, acc, corner_of(P, s.lower, k), first + second);
                                 }
                             }
                         } else {
@@ -1588,7 +1606,7 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
                                     double contribution = vf;
 #pragma unroll
                                     for (int a = 0; a < 3; ++a) contribution *= ((k >> a) & 1) ? f[a] : 1.0 - f[a];
-                                    bad |= !add_increment(acc, corner_of(P, lower, k), contribution);
+                                    bad |= !add_increment(P, acc, corner_of(P, lower, k), contribution);
                                 }
                             }
                         }
@@ -1711,15 +1729,19 @@ __global__ void fill_i32_kernel(int *__restrict__ x, int n) {
     if (i < n) x[i] = 0;
 }
 
-// FP64 accumulator -> FP32 gradient (write or accumulate) + finite check
-// (kernels/projection_checks.py:49-58 applied to the final gradient).
+// FP64 accumulator -> FP32 or FP64 gradient (write or accumulate) + finite
+// check (kernels/projection_checks.py:49-58 applied to the final gradient).
+template <typename G>
 __global__ __launch_bounds__(kBlock) void finish_volume_kernel(const double *__restrict__ acc,
-                                                               float *__restrict__ grad, int64_t n,
+                                                               G *__restrict__ grad, int64_t n,
                                                                int accumulate, int *status) {
     bool bad = false;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        float v = (float)acc[i];
-        if (accumulate) v += grad[i];
+        // the previous value joins the FP64 sum: one rounding to G, and no
+        // FP32 overflow of a sum that the previous value brings back in range
+        double sum = acc[i];
+        if (accumulate) sum += (double)grad[i];
+        const G v = (G)sum;
         grad[i] = v;
         bad |= !isfinite(v);
     }
@@ -1998,10 +2020,11 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, b
     return DPT_OK;
 }
 
-// Pose partials, or the caller's per-ray buffer for ray coordinates
+// Pose partials, or the caller's per-ray buffer for ray coordinates and
+// per-ray tangents
 template <int C>
 double *pose_destination(dpt_plan *p) {
-    return C == kRayCoordinates ? p->ray_out : p->d_partials;
+    return (C == kRayCoordinates || C == kRayTangent) ? p->ray_out : p->d_partials;
 }
 
 template <typename T, int C, int LAYOUT>
@@ -2075,6 +2098,7 @@ template <typename T>
 int launch_vjp(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
                int components, bool cells, hipStream_t s) {
     if (components == kRayCoordinates) return launch_vjp_c<T, kRayCoordinates>(p, P, field, seeds, false, true, cells, s);
+    if (components == kRayTangent) return launch_vjp_c<T, kRayTangent>(p, P, field, seeds, false, true, cells, s);
     if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, cells, s);
     return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, cells, s);
 }
@@ -2384,7 +2408,7 @@ int dpt_forward(dpt_plan *p, const void *field, const double *poses, void *out, 
     return rc;
 }
 
-int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, float *grad_field,
+int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, void *grad_field,
             double *grad_pose, uint32_t flags, void *stream) {
     if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr)
         return fail(DPT_E_INVALID, "plan, field, poses and cotangent are required");
@@ -2394,10 +2418,15 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
         return fail(DPT_E_INVALID, "volume gradient requested without a destination or plan support");
     if (pose && (grad_pose == nullptr || !(p->needs & DPT_POSE)))
         return fail(DPT_E_INVALID, "pose gradient requested without a destination or plan support");
+    if (flags & ~(uint32_t)(DPT_VOLUME | DPT_POSE | DPT_POSE_TANGENT | DPT_ACCUMULATE | DPT_NO_CHECKS |
+                            DPT_VOLUME_FP64))
+        return fail(DPT_E_INVALID, "unknown VJP flags");
+    const bool vol64 = vol && (flags & DPT_VOLUME_FP64);
     const int components = (flags & DPT_POSE_TANGENT) ? 6 : 12;
     DeviceGuard guard(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     Params P = launch_params(p, poses);
+    P.volume_f64 = vol64 ? 1 : 0;
     configure_tiles(P, p->vjp_vb);
     if (!(flags & DPT_NO_CHECKS)) {
         switch (p->storage) {
@@ -2451,18 +2480,25 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
             hipLaunchKernelGGL(pose_reduce_kernel<12>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
                                P.wt_per_view, grad_pose, accumulate, p->d_status);
     }
-    if (vol)
-        hipLaunchKernelGGL(finish_volume_kernel, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s, p->d_acc,
-                           grad_field, p->voxels, accumulate, p->d_status);
+    if (vol64)
+        hipLaunchKernelGGL(finish_volume_kernel<double>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                           p->d_acc, static_cast<double *>(grad_field), p->voxels, accumulate, p->d_status);
+    else if (vol)
+        hipLaunchKernelGGL(finish_volume_kernel<float>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,
+                           p->d_acc, static_cast<float *>(grad_field), p->voxels, accumulate, p->d_status);
     HIP_TRY(hipGetLastError());
     return DPT_OK;
 }
 
-int dpt_ray_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, double *out_rays,
-                uint32_t flags, void *stream) {
-    if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr || out_rays == nullptr)
-        return fail(DPT_E_INVALID, "plan, field, poses, cotangent and out_rays are required");
-    if (flags & ~(uint32_t)DPT_NO_CHECKS) return fail(DPT_E_INVALID, "ray-coordinate VJP flags are DPT_NO_CHECKS");
+}  // extern "C"
+
+namespace {
+
+// Per-ray VJP outputs (6 FP64 values per ray into the caller's buffer):
+// mode kRayCoordinates (world source / endpoint derivatives) or kRayTangent
+// (6-DoF local-right pose tangent).
+int per_ray_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, double *out_rays,
+                int mode, uint32_t flags, void *stream) {
     DeviceGuard guard(p->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     Params P = launch_params(p, poses);
@@ -2498,13 +2534,33 @@ int dpt_ray_vjp(dpt_plan *p, const void *field, const double *poses, const void 
     p->ray_out = out_rays;
     int rc;
     if (p->storage == DPT_STORE_F32)
-        rc = launch_vjp<float>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+        rc = launch_vjp<float>(p, P, field, cotangent, false, true, mode, p->cells, s);
     else if (p->storage == DPT_STORE_BF16)
-        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+        rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, false, true, mode, p->cells, s);
     else
-        rc = launch_vjp<__half>(p, P, field, cotangent, false, true, kRayCoordinates, p->cells, s);
+        rc = launch_vjp<__half>(p, P, field, cotangent, false, true, mode, p->cells, s);
     p->ray_out = nullptr;
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpt_ray_vjp(dpt_plan *p, const void *field, const double *poses, const void *cotangent, double *out_rays,
+                uint32_t flags, void *stream) {
+    if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr || out_rays == nullptr)
+        return fail(DPT_E_INVALID, "plan, field, poses, cotangent and out_rays are required");
+    if (flags & ~(uint32_t)DPT_NO_CHECKS) return fail(DPT_E_INVALID, "ray-coordinate VJP flags are DPT_NO_CHECKS");
+    return per_ray_vjp(p, field, poses, cotangent, out_rays, kRayCoordinates, flags, stream);
+}
+
+int dpt_pose_sensitivities(dpt_plan *p, const void *field, const double *poses, const void *cotangent,
+                           double *out_jacobian, uint32_t flags, void *stream) {
+    if (p == nullptr || field == nullptr || poses == nullptr || cotangent == nullptr || out_jacobian == nullptr)
+        return fail(DPT_E_INVALID, "plan, field, poses, cotangent and out_jacobian are required");
+    if (flags & ~(uint32_t)DPT_NO_CHECKS) return fail(DPT_E_INVALID, "pose sensitivity flags are DPT_NO_CHECKS");
+    return per_ray_vjp(p, field, poses, cotangent, out_jacobian, kRayTangent, flags, stream);
 }
 
 }  // extern "C"

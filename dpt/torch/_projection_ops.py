@@ -193,6 +193,26 @@ def ray_vjp(field: torch.Tensor, poses: torch.Tensor, cotangent: torch.Tensor, v
     return out
 
 
+def pose_sensitivities(field: torch.Tensor, poses: torch.Tensor, cotangent: torch.Tensor, value: str) -> torch.Tensor:
+    """Per ray: cotangent x d(optical depth)/d(delta) for pose @ exp(delta^),
+    shape (views, rows, columns, 6) float64, rows ordered tx, ty, tz, rx, ry,
+    rz (projection_pose_sensitivities, python/dpt/projection.py:1021-1084)."""
+    shape, dtype = _metadata(field, poses, value)
+    if tuple(cotangent.shape) != shape or cotangent.dtype != dtype or cotangent.device != field.device:
+        raise ContractError("cotangent must match the projected image shape, dtype and device")
+    _require_device(field)
+    stream = torch.cuda.current_stream(field.device).cuda_stream
+    entry = prepare(value, field.device.index or 0, stream, 0)
+    with entry.lock, torch.no_grad():
+        packed = poses.contiguous()
+        seed = cotangent.contiguous()
+        out = torch.empty((*shape, 6), dtype=torch.float64, device=field.device)
+        pointer = _field_pointer(entry, field, stream)
+        entry.plan.pose_sensitivities(pointer, packed.data_ptr(), seed.data_ptr(), out.data_ptr(), 0, stream)
+        consume(entry, stream)
+    return out
+
+
 def reduce_rays(rays: torch.Tensor, jacobians: torch.Tensor, value: str) -> torch.Tensor:
     """out[v, k] = sum_p rays[v, p, :] . jacobians[v, k, :] in a fixed order
     (the intrinsic reduction of batched_projection_vjp,

@@ -201,6 +201,22 @@ class Projector(torch.nn.Module):
         poses = self._poses(field, pose, packed_pose).detach()
         return _projection_ops.ray_vjp(field.detach(), poses.contiguous(), cotangent, self._configuration)
 
+    def pose_sensitivities(self, field: torch.Tensor, cotangent: torch.Tensor | None = None,
+                           pose: torch.Tensor | None = None, *, packed_pose=None) -> torch.Tensor:
+        """Per ray: cotangent x d(optical depth)/d(delta) for G @ exp(delta^),
+        G the packed pose this module evaluates, shape (views, rows, columns,
+        6) float64 ordered tx, ty, tz, rx, ry, rz in mm and rad; without a
+        cotangent the rows are the unweighted Jacobian.  Missed rays give zero
+        rows (projection_pose_sensitivities,
+        python/dpt/projection.py:1021-1084)."""
+        if not isinstance(field, torch.Tensor) or tuple(field.shape) != self.grid.shape:
+            raise ContractError("field must be a Torch tensor with grid shape (nz,ny,nx)")
+        poses = self._poses(field, pose, packed_pose).detach()
+        if cotangent is None:
+            dtype = torch.float32 if self.precision == "fp32" else torch.float64
+            cotangent = torch.ones(self.image_shape, dtype=dtype, device=field.device)
+        return _projection_ops.pose_sensitivities(field.detach(), poses.contiguous(), cotangent, self._configuration)
+
     def intrinsics_vjp(self, field: torch.Tensor, cotangent: torch.Tensor, intrinsics, angles_rad,
                        pose: torch.Tensor | None = None, *, packed_pose=None) -> torch.Tensor:
         """Cone intrinsic cotangent (views, 4) = SOD, SDD, column offset, row

@@ -73,6 +73,10 @@ extern "C" {
 #define DPT_POSE_TANGENT 4u  /* 6-component local-right tangent instead of 12 */
 #define DPT_ACCUMULATE 8u    /* add into grad_pose / grad_field instead of writing */
 #define DPT_NO_CHECKS 16u    /* skip input validation (caller validated once) */
+/* grad_field is float64 (volume_gradient_precision="float64",
+ * python/dpt/projection.py:871-1018): FP32-sampled increments RN(f * w) are
+ * not flushed, FP64-sampled ones are not rounded to FP32 */
+#define DPT_VOLUME_FP64 32u
 
 /* status bits (python/dpt/contracts.py:229-234, projection.py:505-515) */
 #define DPT_STATUS_POSE 1    /* invalid rigid pose -> ContractError */
@@ -149,15 +153,16 @@ int dpt_forward(dpt_plan *plan, const void *field, const double *poses, void *ou
 
 /*
  * Vector-Jacobian product for a cotangent (views, rows, cols) of the output
- * dtype.  grad_field: float32 (nz,ny,nx) (DPT_VOLUME); grad_pose: float64
- * (views, 12) or (views, 6) with DPT_POSE_TANGENT (DPT_POSE).
+ * dtype.  grad_field: float32 (nz,ny,nx) (DPT_VOLUME), float64 with
+ * DPT_VOLUME_FP64; grad_pose: float64 (views, 12) or (views, 6) with
+ * DPT_POSE_TANGENT (DPT_POSE).
  * Replaces project_vjp (python/dpt/torch/_projection_ops.py:273-353) around
  * projection_vjp (python/dpt/projection.py:871-1018), the VJP kernels
  * kernels/projection_fp32.py:377-617 / kernels/projection.py:873-1237 and the
  * reduction tree (python/dpt/_reductions.py:27-86).
  */
 int dpt_vjp(dpt_plan *plan, const void *field, const double *poses, const void *cotangent,
-            float *grad_field, double *grad_pose, uint32_t flags, void *stream);
+            void *grad_field, double *grad_pose, uint32_t flags, void *stream);
 
 /*
  * Ray-coordinate VJP: out_rays (views, rows, cols, 6) float64 = cotangent[p]
@@ -171,6 +176,21 @@ int dpt_vjp(dpt_plan *plan, const void *field, const double *poses, const void *
  */
 int dpt_ray_vjp(dpt_plan *plan, const void *field, const double *poses, const void *cotangent,
                 double *out_rays, uint32_t flags, void *stream);
+
+/*
+ * Per-ray pose sensitivities: out_jacobian (views, rows, cols, 6) float64 =
+ * cotangent[p] x d(optical depth p)/d(delta) for pose @ exp(delta^), rows
+ * ordered tx, ty, tz, rx, ry, rz (mm, rad), one row per ray with no
+ * reduction; unit cotangents export the unweighted Jacobian, missed rays and
+ * zero cotangents give zero rows.  Through the plan's arithmetic and
+ * integration rule (the clip-face and interpolation branches included).
+ * cotangent as for dpt_vjp; flags: DPT_NO_CHECKS.  Replaces
+ * projection_pose_sensitivities (python/dpt/projection.py:1021-1084), the
+ * per_ray branch of the VJP kernels (kernels/projection_fp32.py:377-617,
+ * kernels/projection.py:873-1237).
+ */
+int dpt_pose_sensitivities(dpt_plan *plan, const void *field, const double *poses, const void *cotangent,
+                           double *out_jacobian, uint32_t flags, void *stream);
 
 /*
  * out (views, components) float64 = per view, sum over its pixels of

@@ -482,58 +482,100 @@ def integrate_segment(values, grid: Grid, pose, source, endpoint, midpoint_sampl
     return length * math.fsum(total)
 
 
-def explicit_fp32_matrix(grid: Grid, segments, pose, samples: int) -> np.ndarray:
-    """Rounded-coordinate FP32 interpolation matrix, (rays, voxels) in FP64.
-
-    Restates tests/python/warp_cpu/test_projection_fp32.py:71-119: FP32 ray
+def _fp32_ray_samples(grid: Grid, pose, source, endpoint, samples: int):
+    """One ray of the rounded-coordinate FP32 walk: (physical step, [(voxel,
+    weight) per inside sample and corner]) or None for a miss.  FP32 ray
     coordinates and clipping, FP32 midpoints via an exact-product FMA, FP32
-    weights; each entry is RN32(physical_step * weight) summed in FP64.
-    """
+    weights (tests/python/warp_cpu/test_projection_fp32.py:71-119)."""
     f32 = np.float32
 
     def fma32(a, b, c):
         return f32(float(a) * float(b) + float(c))
 
     shape = grid.xyz
+    origin, direction, _, _, length = clip_segment(grid, pose, source, endpoint)
+    origin = np.asarray(origin, dtype=f32)
+    direction = np.asarray(direction, dtype=f32)
+    lower, upper = f32(0), f32(1)
+    for axis in range(3):
+        low, high = f32(-0.5), f32(shape[axis] - 0.5)
+        velocity = direction[axis]
+        if velocity == 0:
+            if origin[axis] < low or origin[axis] > high:
+                return None
+        else:
+            first = f32(f32(low - origin[axis]) / velocity)
+            last = f32(f32(high - origin[axis]) / velocity)
+            first, last = min(first, last), max(first, last)
+            lower, upper = max(lower, first), min(upper, last)
+    if upper <= lower:
+        return None
+    step = f32(f32(upper - lower) / f32(samples))
+    physical = f32(f32(length) * step)
+    terms = []
+    for index in range(samples):
+        position = fma32(f32(index) + f32(0.5), step, lower)
+        point = [fma32(position, direction[a], origin[a]) for a in range(3)]
+        if any(point[a] < -0.5 or point[a] > shape[a] - 0.5 for a in range(3)):
+            continue
+        vals = [f32(min(max(point[a], 0), shape[a] - 1)) for a in range(3)]
+        cell = [min(math.floor(vals[a]), max(shape[a] - 2, 0)) for a in range(3)]
+        fractions = [f32(vals[a] - f32(cell[a])) for a in range(3)]
+        for corner in range(8):
+            idx = [min(cell[a] + ((corner >> a) & 1), shape[a] - 1) for a in range(3)]
+            factors = [fractions[a] if (corner >> a) & 1 else f32(1) - fractions[a] for a in range(3)]
+            weight = f32(f32(factors[0] * factors[1]) * factors[2])
+            terms.append(((idx[2] * shape[1] + idx[1]) * shape[0] + idx[0], weight))
+    return physical, terms
+
+
+def explicit_fp32_matrix(grid: Grid, segments, pose, samples: int) -> np.ndarray:
+    """Rounded-coordinate FP32 interpolation matrix, (rays, voxels) in FP64.
+
+    Restates tests/python/warp_cpu/test_projection_fp32.py:71-119: each entry
+    is RN32(physical_step * weight) summed in FP64.
+    """
+    f32 = np.float32
     matrix = np.zeros((len(segments), grid.voxels), dtype=np.float64)
     for ray, (source, endpoint) in enumerate(segments):
-        origin, direction, _, _, length = clip_segment(grid, pose, source, endpoint)
-        origin = np.asarray(origin, dtype=f32)
-        direction = np.asarray(direction, dtype=f32)
-        lower, upper = f32(0), f32(1)
-        miss = False
-        for axis in range(3):
-            low, high = f32(-0.5), f32(shape[axis] - 0.5)
-            velocity = direction[axis]
-            if velocity == 0:
-                if origin[axis] < low or origin[axis] > high:
-                    miss = True
-            else:
-                first = f32(f32(low - origin[axis]) / velocity)
-                last = f32(f32(high - origin[axis]) / velocity)
-                first, last = min(first, last), max(first, last)
-                lower, upper = max(lower, first), min(upper, last)
-        if miss or upper <= lower:
+        walked = _fp32_ray_samples(grid, pose, source, endpoint, samples)
+        if walked is None:
             continue
-        step = f32(f32(upper - lower) / f32(samples))
-        physical = f32(f32(length) * step)
-        for index in range(samples):
-            position = fma32(f32(index) + f32(0.5), step, lower)
-            point = [fma32(position, direction[a], origin[a]) for a in range(3)]
-            if any(point[a] < -0.5 or point[a] > shape[a] - 0.5 for a in range(3)):
-                continue
-            vals = [f32(min(max(point[a], 0), shape[a] - 1)) for a in range(3)]
-            cell = [min(math.floor(vals[a]), max(shape[a] - 2, 0)) for a in range(3)]
-            fractions = [f32(vals[a] - f32(cell[a])) for a in range(3)]
-            for corner in range(8):
-                idx = [min(cell[a] + ((corner >> a) & 1), shape[a] - 1) for a in range(3)]
-                factors = [
-                    fractions[a] if (corner >> a) & 1 else f32(1) - fractions[a] for a in range(3)
-                ]
-                weight = f32(f32(factors[0] * factors[1]) * factors[2])
-                voxel = (idx[2] * shape[1] + idx[1]) * shape[0] + idx[0]
-                matrix[ray, voxel] += float(f32(physical * weight))
+        physical, terms = walked
+        for voxel, weight in terms:
+            matrix[ray, voxel] += float(f32(physical * weight))
     return matrix
+
+
+def explicit_fp32_scatter(grid: Grid, segments, pose, samples: int, seeds, *, flush: bool = True) -> np.ndarray:
+    """Per-sample FP32 scatter of cotangents, summed in FP64 (voxels,).
+
+    Restates the per-sample reference scatter of
+    tests/python/warp_cpu/test_projection_cell_aggregation.py:22-48 with the
+    GPU increment rule of kernels/projection_fp32.py:240-252: factor =
+    RN32(seed * physical_step), increment RN32(factor * weight), flushed to a
+    signed zero below 2^-126 when ``flush`` (the reference's GPU path;
+    ``flush=False`` is its FP64-destination scatter, :212-235).
+    """
+    f32 = np.float32
+    tiny = float(np.finfo(np.float32).tiny)
+    out = np.zeros(grid.voxels, dtype=np.float64)
+    for (source, endpoint), seed in zip(segments, seeds):
+        seed = f32(seed)
+        if seed == 0:
+            continue
+        walked = _fp32_ray_samples(grid, pose, source, endpoint, samples)
+        if walked is None:
+            continue
+        physical, terms = walked
+        with np.errstate(over="ignore"):
+            factor = f32(seed * physical)
+            for voxel, weight in terms:
+                increment = f32(factor * weight)
+                if flush and abs(float(increment)) < tiny:
+                    increment = f32(0.0)
+                out[voxel] += float(increment)
+    return out
 
 
 def ellipsoid_chord(axes, mu, centre, source, target) -> float:

@@ -190,3 +190,41 @@ def curved_segment(p: dict, row: int, col: int):
     end = tuple(p["origin_mm"][i] + (math.cos(angle) - 1.0) * radial[i] + math.sin(angle) * radius * p["u"][i]
                 + row * p["spacing_mm"][1] * p["v"][i] for i in range(3))
     return tuple(p["source_mm"]), end
+
+
+# tests/python/warp_cpu/test_projection_shared_face.py:91-153: rays that walk
+# the cells of small grids along and across the axes, seeds scaled by 1 and
+# 1000 (cell sums that carry across shared faces), overflow-scale seeds
+# accumulated onto an opposing destination, and subnormal-scale seeds whose
+# every increment the GPU rule flushes to zero.
+SHARED_FACE_DIRECTIONS = ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, -1))
+
+
+def shared_face_scene(direction):
+    """A 5x5x5 unit grid and three rays through its centre along +-direction
+    (plus one that misses), field linspace(0.01, 0.1)."""
+    centre = np.asarray([2.0, 2.0, 2.0])
+    displacement = np.asarray(direction, dtype=np.float64) * 10.0
+    source, target = centre - displacement, centre + displacement
+    sources = np.array([source, target, source + 100.0])
+    endpoints = np.array([target, source, target + 100.0])
+    grid = O.Grid((5, 5, 5), (1.0, 1.0, 1.0))
+    return grid, sources, endpoints, np.linspace(0.01, 0.1, grid.voxels, dtype=np.float32)
+
+
+def overflow_scene(reverse: bool):
+    """Grid (2, 2, 3) at (2, 1, 1) mm and one ray along x: each grid sample
+    it passes collects about 2 mm x seed."""
+    source, target = np.array([[-3.0, 0.0, 0.0]]), np.array([[7.0, 0.0, 0.0]])
+    if reverse:
+        source, target = target, source
+    grid = O.Grid((2, 2, 3), (2.0, 1.0, 1.0))
+    return grid, source, target, np.linspace(0.01, 0.1, grid.voxels, dtype=np.float32)
+
+
+def subnormal_scene():
+    """Grid (2, 2, 3) of unit spacing and one ray along x at y = z = 0.5: with
+    a seed of 1e-35 every increment factor x weight is below 2^-126."""
+    grid = O.Grid((2, 2, 3), (1.0, 1.0, 1.0))
+    return (grid, np.array([[-2.0, 0.5, 0.5]]), np.array([[5.0, 0.5, 0.5]]),
+            np.linspace(0.01, 0.1, grid.voxels, dtype=np.float32))

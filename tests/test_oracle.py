@@ -457,3 +457,51 @@ def test_ray_coordinate_vjp_matches_central_differences(mode):
             fd = seeds[ray] * (depth(h) - depth(-h)) / (2 * h)
             scale = max(1e-3, np.abs(rays[ray]).max())
             assert abs(rays[ray, j] - fd) <= tol * scale, (ray, j, rays[ray, j], fd)
+
+
+@pytest.mark.parametrize("seed_scale", [1.0, 1000.0])
+@pytest.mark.parametrize("samples", [2, 7, 1024])
+@pytest.mark.parametrize("direction", scenes.SHARED_FACE_DIRECTIONS)
+def test_fp32_volume_vjp_matches_independent_per_sample_scatter(direction, samples, seed_scale):
+    """tests/python/warp_cpu/test_projection_shared_face.py:91-112 on the oracle."""
+    grid, sources, endpoints, values = scenes.shared_face_scene(direction)
+    seeds = (seed_scale * np.asarray([0.7, -0.3, 0.0])).astype(np.float32)
+    scene = O.Scene(grid, samples, rays=(sources, endpoints))
+    volume, _, _, status = scene.vjp(values, seeds, pose=False)
+    expected = O.explicit_fp32_scatter(grid, list(zip(sources, endpoints)), scenes.IDENTITY12, samples, seeds)
+    assert status.tolist() == [0]
+    assert np.count_nonzero(expected) > 0
+    np.testing.assert_allclose(volume, expected, rtol=2e-5, atol=3e-7 * seed_scale)
+
+
+@pytest.mark.parametrize("sign", [-1.0, 1.0])
+def test_subnormal_increments_flush_to_exact_zeros(sign):
+    """tests/python/warp_cpu/test_projection_shared_face.py:138-153: a seed of
+    +-1e-35 makes every increment subnormal; the GPU rule gives exact zeros
+    while the unflushed products do not (the planted mistake)."""
+    grid, source, target, values = scenes.subnormal_scene()
+    seeds = np.asarray([sign * 1e-35], dtype=np.float32)
+    volume, _, _, status = O.Scene(grid, 1024, rays=(source, target)).vjp(values, seeds, pose=False)
+    assert status.tolist() == [0]
+    assert np.all(volume == 0.0)
+    segments = list(zip(source, target))
+    np.testing.assert_array_equal(O.explicit_fp32_scatter(grid, segments, scenes.IDENTITY12, 1024, seeds), volume)
+    unflushed = O.explicit_fp32_scatter(grid, segments, scenes.IDENTITY12, 1024, seeds, flush=False)
+    assert np.count_nonzero(unflushed) > 0 and np.all(np.sign(unflushed[unflushed != 0]) == sign)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("sign", [-1.0, 1.0])
+def test_overflow_scale_seeds_stay_finite_on_an_opposing_destination(sign, reverse):
+    """tests/python/warp_cpu/test_projection_shared_face.py:115-135: about
+    4e38 per grid sample (beyond FLT_MAX) added to -3e38 gives a finite
+    +1e38 at grid sample 1; the oracle's FP64 sums carry no overflow."""
+    grid, source, target, values = scenes.overflow_scene(reverse)
+    seeds = np.asarray([sign * 2e38], dtype=np.float32)
+    volume, _, _, status = O.Scene(grid, 1024, rays=(source, target)).vjp(values, seeds, pose=False)
+    assert status.tolist() == [0]
+    expected = O.explicit_fp32_scatter(grid, list(zip(source, target)), scenes.IDENTITY12, 1024, seeds)
+    np.testing.assert_allclose(volume, expected, rtol=2e-5, atol=0)
+    total = volume + np.full(grid.voxels, -sign * 3e38)
+    assert np.all(np.isfinite(total.astype(np.float32)))
+    np.testing.assert_allclose(total[1], sign * 1e38, rtol=2e-5)
