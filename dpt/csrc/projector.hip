@@ -156,13 +156,13 @@ __device__ __forceinline__ void store_gradient(const Params &P, const double *g,
     }
 }
 
-// Register cell accumulator for the volume cotangent (reference Vec8 sums,
-// kernels/projection_fp32.py:261-269): increments of consecutive samples in one
-// interpolation cell are summed in FP32 registers, then flushed once.
+// Register cell accumulator of the volume cotangent (the reference's Vec8
+// cell sums, kernels/projection_fp32.py:261-269): a lane sums f * w over its
+// consecutive samples in one interpolation cell, one FP32 fused multiply-add
+// per corner and sample, and flushes the cell once when it leaves it.
 struct CellSums {
     int x, y, z;   // lower corner; x < 0 when empty
-    float s[8];    // FP32 sums of the corner weights of the cell's samples
-    float scale;   // the ray's volume factor: a flushed corner adds scale * s[k]
+    float s[8];    // FP32 sums of f * w of the cell's samples
     __device__ __forceinline__ void reset() {
         x = -1;
 #pragma unroll
@@ -180,104 +180,27 @@ struct Brick {
     int origin;          // (z0 * by + y0) * bx + x0: local index = (z * by + y) * bx + x - origin
 };
 
-// Add the values f[k] at the eight corners of the register cell: into the
-// LDS brick (FP64 LDS atomics, every corner: a lane's cell sums are non-zero
-// but for exact-boundary fractions) or into the FP64 global accumulator
-// (non-zero values only: device atomics are the expensive kind).
-template <bool LDS>
-__device__ __forceinline__ void add_corners(const Params &P, const Brick &B, const CellSums &cs, const float *f,
-                                            double *brick, double *acc) {
-    if constexpr (LDS) {
-        // (z * by + y) * bx + x < ny * nz * nx: operands below 2^24 (grid check)
-        const int base =
-            (int)(__umul24(__umul24((unsigned)cs.z, (unsigned)B.by) + (unsigned)cs.y, (unsigned)B.bx) + (unsigned)cs.x) -
-            B.origin;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
-            atomicAdd(&brick[base + off], (double)cs.scale * (double)f[k]);
-        }
-    } else {
-        const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (f[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.scale * (double)f[k]);
-    }
-}
-
-template <bool LDS>
-__device__ __forceinline__ void flush(const Params &P, const Brick &B, CellSums &cs, double *brick, double *acc) {
-    if (cs.x < 0) return;
-    add_corners<LDS>(P, B, cs, cs.s, brick, acc);
-    cs.reset();
-}
-
-// Cell transition with the shared-face carry of the reference
-// (kernels/projection_fp32.py:272-331): when the new cell is the old one
-// stepped by +-1 along one axis, the four corners on the shared face stay in
-// registers (renamed to the opposite side of the new cell) and only the four
-// departing corners are flushed; any other move flushes all eight.  Used for
-// chunks whose brick does not fit LDS, whose flushes are device atomics (the
-// expensive kind): the carry halves them.
-//
-// Inside a lockstep sample loop some lane of a wave changes cell at nearly
-// every sample, so this code runs at wave level once per sample: it is
-// written as lane masks and selects (no data-dependent branches besides the
-// per-corner atomic masks).  An empty register cell (x < 0) holds zeros, so
-// it flushes nothing and carries zeros.
-template <bool LDS>
-__device__ __forceinline__ void transition_carry(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
-                                                 double *brick, double *acc) {
-    const int dx = c.lx - cs.x, dy = c.ly - cs.y, dz = c.lz - cs.z;
-    const bool live = cs.x >= 0;
-    const bool on_x = live && dy == 0 && dz == 0, on_y = live && dx == 0 && dz == 0,
-               on_z = live && dx == 0 && dy == 0;
-    // +1 / -1 unit steps along each axis; at most one is set
-    const bool px = on_x && dx == 1, mx = on_x && dx == -1;
-    const bool py = on_y && dy == 1, my = on_y && dy == -1;
-    const bool pz = on_z && dz == 1, mz = on_z && dz == -1;
-    float departing[8], carried[8];
+// Add a cell's sums to the LDS brick: eight FP64 LDS atomics (every corner;
+// exact-boundary fractions aside, a visited cell's sums are non-zero).
+__device__ __forceinline__ void flush_lds(const Brick &B, const CellSums &cs, double *brick) {
+    // (z * by + y) * bx + x < ny * nz * nx: operands below 2^24 (grid check)
+    const int base =
+        (int)(__umul24(__umul24((unsigned)cs.z, (unsigned)B.by) + (unsigned)cs.y, (unsigned)B.bx) + (unsigned)cs.x) -
+        B.origin;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        // old corner k lies on the shared face: its bit along the step axis is
-        // set for a +1 step and clear for a -1 step
-        const bool shared = ((k & 1) ? px : mx) || ((k & 2) ? py : my) || ((k & 4) ? pz : mz);
-        departing[k] = shared ? 0.0f : cs.s[k];
+        const int off = ((k & 1) ? B.lsx : 0) + ((k & 2) ? B.lsy : 0) + ((k & 4) ? B.lsz : 0);
+        atomicAdd(&brick[base + off], (double)cs.s[k]);
     }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        // new corner j is old corner j ^ bit when that corner is shared
-        carried[j] = ((j & 1) ? mx : px)   ? cs.s[j ^ 1]
-                     : ((j & 2) ? my : py) ? cs.s[j ^ 2]
-                     : ((j & 4) ? mz : pz) ? cs.s[j ^ 4]
-                                           : 0.0f;
-    }
-    add_corners<LDS>(P, B, cs, departing, brick, acc);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) cs.s[j] = carried[j];
-    cs.x = c.lx;
-    cs.y = c.ly;
-    cs.z = c.lz;
 }
 
-// Cell change.  Into an LDS brick the register cell sums
-// (kernels/projection_fp32.py:261-269) are flushed whole and the new cell
-// starts from zero: the carry's 24 selects and renames, run at wave level
-// once per sample, cost a third of the kernel's VALU, while the FP64 LDS
-// atomics it saves are cheap.  The result differs from the carried form
-// only in how the FP32 partial sums of a shared corner are split before
-// their exact FP64 addition.
-template <bool LDS>
-__device__ __forceinline__ void transition(const Params &P, const Brick &B, CellSums &cs, const Cell32 &c,
-                                           double *brick, double *acc) {
-    if constexpr (LDS) {
-        flush<true>(P, B, cs, brick, acc);
-        cs.x = c.lx;
-        cs.y = c.ly;
-        cs.z = c.lz;
-    } else {
-        transition_carry<false>(P, B, cs, c, brick, acc);
-    }
+// Add a cell's non-zero sums to the FP64 device accumulator (chunks whose
+// brick does not fit LDS).
+__device__ __forceinline__ void flush_device(const Params &P, const CellSums &cs, double *acc) {
+    const int base = (cs.z * P.ny + cs.y) * P.nx + cs.x;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (cs.s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.s[k]);
 }
 
 // Samples per lockstep chunk of the volume adjoint: 32 at 1024 samples over
@@ -295,6 +218,9 @@ inline int chunk_samples(int samples, int nx, int ny, int nz) {
 // cycles per wave64 instruction, measured), so the brick accumulates the FP32
 // cell sums in FP64.
 constexpr int kBrickCapDefault = 4096;
+// Shortest chunk the FP32 volume adjoint halves an overflowing chunk to
+// before it flushes to device atomics.
+constexpr int kMinChunk = 4;
 
 // Host estimate of the chunk bricks of the first view at the identity pose:
 // the grid samples touched by 16 x 16 pixel tiles (every 4th tile along
@@ -478,7 +404,7 @@ __device__ __forceinline__ int wave_min(int v) {
 // negated maximum over its chunk samples (INT_MAX when it has none).  Every
 // lane gets the same answer (kept in SGPRs); one barrier.
 __device__ __forceinline__ bool chunk_brick(const Params &P, int (*red)[6], int *lo, int *nhi, bool unit_x,
-                                            Brick &B) {
+                                            Brick &B, bool *overflow = nullptr) {
     const int wv = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -505,6 +431,7 @@ __device__ __forceinline__ bool chunk_brick(const Params &P, int (*red)[6], int 
 #pragma unroll
     for (int a = 0; a < 3; ++a) e[a] = min(h[a] + 1, n[a] - 1) - l[a] + 1;
     const long long vol = (long long)e[0] * e[1] * e[2];
+    if (overflow != nullptr) *overflow = vol > P.brick_cap;
     if (vol > P.brick_cap) return false;
     // every lane holds the same values: keep them in SGPRs
     B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
@@ -551,7 +478,26 @@ __device__ __forceinline__ void drain_brick(const Params &P, const Brick &B, dou
 // the chunk bound each ray), privatises that brick in LDS when it fits, sums
 // register cell accumulators into it with LDS atomics, then adds the non-zero
 // brick entries to the FP64 accumulator with one global atomic each.  Chunks
-// whose brick does not fit use global atomics directly.
+// whose brick does not fit flush cells with global atomics directly.
+//
+// Volume factors f = seed * length * step.  Rays with 2^-102 <= |f| <= 2^96
+// accumulate RN(fma(w, f, s)) per cell in the sample loop: against the
+// reference's RN(f * w) summed in FP32 this drops one rounding per sample,
+// and the reference's flush of increments below 2^-126
+// (kernels/projection_fp32.py:240-252) can only concern terms with
+// w < 2^-24, each below 2^-126; the bound 2^96 keeps every FP32 cell sum
+// finite.  Other rays (smaller factors, whose increments the reference
+// flushes to exact zeros, and larger or non-finite ones) add the reference's
+// per-increment RN(f * w) (flushed, or unflushed for an FP64 destination)
+// with device atomics after the pose epilogue.
+//
+// The sample loop is written for wave-level execution: some lane of a wave
+// changes cell at nearly every sample, so the cell-change branch (corner
+// loads, one cell flush, register reset) runs once per sample for the wave,
+// and everything the loop carries is updated in place.  Samples outside the
+// support (only waves whose chunk ends leave it test per sample) take the
+// clamped cell with zero fractions, a zero factor and zero value and slopes:
+// they add nothing, as the reference skips them.
 
 template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
@@ -566,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     const double *pose = P.poses + 12 * (size_t)view;
 
     // ---- per-ray setup
-    bool active = false, weight_sums = false;
+    bool active = false, summed = false, scattered = false, bad = false;
     float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, len = 0.0f, seed = 0.0f, span = 0.0f, step = 0.0f;
     float volume_factor = 0.0f;
     Interval32 iv{};
@@ -595,19 +541,18 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 active = true;
                 step = span / (float)P.samples;
                 volume_factor = seed * (len * step);
-                // Weight-sum form (cell sums of corner weights, scaled by the
-                // factor at the flush) for finite factors with |f| >= 2^-102.
-                // The reference rounds each increment RN(f * w) and flushes
-                // |RN(f * w)| < 2^-126 to a signed zero
-                // (kernels/projection_fp32.py:240-252); with |f| >= 2^-102
-                // that flush can only drop terms with w < 2^-24, each below
-                // 2^-126 in magnitude.  Smaller factors (where whole rays
-                // flush to exact zeros) and non-finite ones take the
-                // reference's per-increment form below.
-                weight_sums = isfinite(volume_factor) && fabsf(volume_factor) >= 0x1p-102f;
+                // sample points are monotone in the sample index: all are
+                // finite iff the first and the last are
+                const bool finite = finite_span(o, d, iv.lower, step, 0, P.samples);
+                bad = !finite;
+                const float m = fabsf(volume_factor);
+                summed = VOL && finite && m >= 0x1p-102f && m <= 0x1p96f;  // false for NaN
+                scattered = VOL && !summed;
             }
         }
     }
+    // the factor of the in-loop cell sums (0 for rays that add elsewhere)
+    const float loop_factor = summed ? volume_factor : 0.0f;
 
     Acc<COMPENSATED> mean, dor0, dor1, dor2, dd0, dd1, dd2, shl, shu;
     mean.init();
@@ -621,86 +566,97 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     shu.init();
     CellSums cs;
     cs.reset();
-    cs.scale = volume_factor;  // the ray's factor (0 for inactive lanes)
-    bool bad = false;
 
     if constexpr (VOL) {
         for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
     }
-    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
-    for (int ch = 0; ch < chunks; ++ch) {
-        const int k0 = ch * P.chunk_samples;
-        const int k1 = min(k0 + P.chunk_samples, P.samples);
+    // Chunks of P.chunk_samples samples; a chunk whose brick overflows LDS is
+    // halved (down to kMinChunk samples) until it fits, and only then flushes
+    // to device atomics.  Chunk bounds are uniform over the workgroup.
+    for (int k0 = 0, k1 = 0; k0 < P.samples; k0 = k1) {
         Brick B{};
         bool use_lds = false;
+        k1 = min(k0 + P.chunk_samples, P.samples);
         if constexpr (VOL) {
-            // ---- bound the cells of this chunk over the workgroup
-            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
-            if (active && weight_sums) {
-                const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
-                const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
-                int la[3], lb[3];
-                lower_cell32(P, fmaf(pa, d[0], o[0]), fmaf(pa, d[1], o[1]), fmaf(pa, d[2], o[2]), la);
-                lower_cell32(P, fmaf(pb, d[0], o[0]), fmaf(pb, d[1], o[1]), fmaf(pb, d[2], o[2]), lb);
+            for (int len = P.chunk_samples;; len >>= 1) {
+                k1 = min(k0 + len, P.samples);
+                // ---- bound the cells of [k0, k1) over the workgroup
+                int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+                if (active && summed) {
+                    const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
+                    const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
+                    int la[3], lb[3];
+                    lower_cell32(P, fmaf(pa, d[0], o[0]), fmaf(pa, d[1], o[1]), fmaf(pa, d[2], o[2]), la);
+                    lower_cell32(P, fmaf(pb, d[0], o[0]), fmaf(pb, d[1], o[1]), fmaf(pb, d[2], o[2]), lb);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    lo[a] = min(la[a], lb[a]);
-                    nhi[a] = -max(la[a], lb[a]);
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = min(la[a], lb[a]);
+                        nhi[a] = -max(la[a], lb[a]);
+                    }
                 }
+                // x-pair and cell layouts run only on grids with nx > 1: a
+                // compile-time unit x stride lets the LDS atomics of
+                // x-adjacent corners use the instruction's offset field
+                bool overflow = false;
+                use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B, &overflow);
+                if (!overflow || len <= kMinChunk) break;
+                __syncthreads();  // the next bound rewrites bound_red
             }
-            // x-pair and cell layouts run only on grids with nx > 1: a
-            // compile-time unit x stride lets the LDS atomics of x-adjacent
-            // corners use the instruction's offset field
-            use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B);
         }
         if (active) {
-            // Lockstep walk of this chunk with the current cell cached (see
-            // forward32_kernel); cell changes trigger masked loads and the
-            // transition of the register sums.
-            bad |= !finite_span(o, d, iv.lower, step, k0, k1);
             float v[8];
             int cur = -1;
-            // Per-sample support tests only for waves with a ray whose chunk
-            // ends leave the support.  Both choices are uniform over the wave
-            // and stay runtime flags: two specialised copies of this loop
-            // raise the register allocation past four waves per SIMD.
+            // per-sample support tests only for waves with a ray whose chunk
+            // ends leave the support (uniform over the wave)
             const bool checked = __builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, k0, k1)) != 0;
             for (int k = k0; k < k1; ++k) {
-                float pos;
+                const float pos = fmaf(sample_centre(P, k), step, iv.lower);
+                const float p0 = fmaf(pos, d[0], o[0]);
+                const float p1 = fmaf(pos, d[1], o[1]);
+                const float p2 = fmaf(pos, d[2], o[2]);
                 Cell32 c;
-                if (checked)
-                    sample_at<true>(P, o, d, iv.lower, step, k, pos, c);
-                else
-                    sample_at<false>(P, o, d, iv.lower, step, k, pos, c);
-                if (checked) {
-                    if (c.base < 0) {  // outside support: nothing to add, close the cell
-                        if constexpr (VOL) {
-                            if (use_lds)
-                                flush<true>(P, B, cs, brick, acc);
-                            else
-                                flush<false>(P, B, cs, brick, acc);
-                        }
-                        cur = -1;
-                        continue;
-                    }
+                cell32_inside(P, p0, p1, p2, c);
+                float factor = loop_factor, live = 1.0f;
+                if (checked) {  // uniform: outside samples add nothing
+                    const bool inside = inside32(P, p0, p1, p2);
+                    c.fx = inside ? c.fx : 0.0f;
+                    c.fy = inside ? c.fy : 0.0f;
+                    c.fz = inside ? c.fz : 0.0f;
+                    c.slx = inside ? c.slx : 0.0f;
+                    c.sly = inside ? c.sly : 0.0f;
+                    c.slz = inside ? c.slz : 0.0f;
+                    factor = inside ? factor : 0.0f;
+                    live = inside ? 1.0f : 0.0f;
                 }
                 if (c.base != cur) {
                     if constexpr (POSE) load8<T, LAYOUT>(P, field, c.base, v);
                     if constexpr (VOL) {
-                        if (weight_sums) {
+                        if (cs.x >= 0) {
                             if (use_lds)
-                                transition<true>(P, B, cs, c, brick, acc);
+                                flush_lds(B, cs, brick);
                             else
-                                transition<false>(P, B, cs, c, brick, acc);
+                                flush_device(P, cs, acc);
                         }
+#pragma unroll
+                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] = 0.0f;
+                        // lanes that add elsewhere keep an empty cell
+                        cs.x = summed ? c.lx : -1;
+                        cs.y = c.ly;
+                        cs.z = c.lz;
                     }
                     cur = c.base;
                 }
                 float wt[8];
                 weights32(c, wt);
+                if constexpr (VOL) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) cs.s[kk] = fmaf(wt[kk], factor, cs.s[kk]);
+                }
                 if constexpr (POSE) {
                     float g[3];
-                    const float value = interpolate32(wt, v);
+                    // an outside sample's slopes are zero (above); its value
+                    // is zeroed here
+                    const float value = interpolate32(wt, v) * live;
                     slopes32(c, v, g);
                     mean.add(value);
                     dor0.add(g[0]);
@@ -714,31 +670,6 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
                     shl.fma_add(1.0f - alpha, along);
                     shu.fma_add(alpha, along);
-                }
-                if constexpr (VOL) {
-                    if (weight_sums) {
-                        // The cell keeps FP32 sums of the corner weights; the
-                        // ray's factor multiplies each sum once, in FP64, when
-                        // the cell is flushed (instead of rounding seed * w
-                        // per sample and corner: 8 multiplies and the
-                        // flush-to-zero test fewer per sample, a difference
-                        // of an FP32 rounding per cell sum).
-#pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] += wt[kk];
-                    } else {
-#pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) {
-                            // FP32 destinations: the reference's flushed
-                            // increment; FP64 ones: RN(f * w) unflushed
-                            // (kernels/projection_fp32.py:212-235)
-                            const float inc = P.volume_f64 ? volume_factor * wt[kk]
-                                                           : scatter_increment(volume_factor, wt[kk]);
-                            if (!isfinite(inc)) bad = true;
-                            // c.base indexes cells in the cell layout: address the voxel
-                            const int voxel = (c.lz * P.ny + c.ly) * P.nx + c.lx;
-                            if (inc != 0.0f) unsafeAtomicAdd(&acc[voxel + corner_offset(P, kk)], (double)inc);
-                        }
-                    }
                 }
             }
         }
@@ -754,17 +685,20 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             shu.fold();
         }
         if constexpr (VOL) {
+            if (cs.x >= 0) {
+                if (use_lds)
+                    flush_lds(B, cs, brick);
+                else
+                    flush_device(P, cs, acc);
+            }
+            cs.reset();
             if (use_lds) {
-                flush<true>(P, B, cs, brick, acc);
                 __syncthreads();
                 drain_brick(P, B, brick, acc);
-            } else {
-                flush<false>(P, B, cs, brick, acc);
             }
             __syncthreads();
         }
     }
-    if (bad) flag(P, view, DPT_STATUS_RANGE);
     if constexpr (POSE) {
         double gradient[kMaxComponents];
 #pragma unroll
@@ -792,6 +726,34 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         }
         store_gradient<C>(P, gradient, partials, wslot, wave_valid, view, row, col);
     }
+    if constexpr (VOL) {
+        // Rays outside the in-loop factor range: the reference's per-increment
+        // RN(f * w) (kernels/projection_fp32.py:240-252, 513-539) per sample
+        // and corner, flushed below 2^-126 for an FP32 destination and kept
+        // for an FP64 one (kernels/projection_fp32.py:212-235), with device
+        // atomics (zeros skipped).
+        if (scattered) {
+            for (int k = 0; k < P.samples; ++k) {
+                const float pos = fmaf(sample_centre(P, k), step, iv.lower);
+                const float p0 = fmaf(pos, d[0], o[0]);
+                const float p1 = fmaf(pos, d[1], o[1]);
+                const float p2 = fmaf(pos, d[2], o[2]);
+                if (!inside32(P, p0, p1, p2)) continue;
+                Cell32 c;
+                cell32_inside(P, p0, p1, p2, c);
+                float wt[8];
+                weights32(c, wt);
+                const int voxel = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) {
+                    const float inc = P.volume_f64 ? volume_factor * wt[kk] : scatter_increment(volume_factor, wt[kk]);
+                    bad |= !isfinite(inc);
+                    if (inc != 0.0f) unsafeAtomicAdd(&acc[voxel + corner_offset(P, kk)], (double)inc);
+                }
+            }
+        }
+    }
+    if (bad) flag(P, view, DPT_STATUS_RANGE);
 }
 
 // --------------------------------------------------------- FP64 sampling --
