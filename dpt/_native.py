@@ -36,6 +36,7 @@ EXPORTS = (
     "dpt_set_timing",
     "dpt_kernel_ms",
     "dpt_count_hit_rays",
+    "dpt_plan_layout",
     "dpt_last_error",
     "dpt_abi_version",
     "dpt_fit_create",
@@ -211,6 +212,8 @@ def lib():
         h.dpt_kernel_ms.argtypes = [vp, i32]
         h.dpt_kernel_ms.restype = ctypes.c_double
         h.dpt_count_hit_rays.argtypes = [vp, vp, vp]
+        h.dpt_plan_layout.argtypes = [vp, vp]
+        h.dpt_plan_layout.restype = ctypes.c_int
         h.dpt_count_hit_rays.restype = i64
         h.dpt_last_error.argtypes = []
         h.dpt_last_error.restype = ctypes.c_char_p
@@ -362,6 +365,14 @@ class Plan:
         if value < 0:
             check(value, "dpt_count_hit_rays")
         return value
+
+    def layout(self) -> dict:
+        """The launch layout the plan chose (brick entries, slab chunks,
+        samples per chunk, views per VJP workgroup)."""
+        out = np.zeros(4, dtype=np.int32)
+        check(lib().dpt_plan_layout(self.handle, out.ctypes.data), "dpt_plan_layout")
+        return {"brick_cap": int(out[0]), "slab_chunks": bool(out[1]), "chunk_samples": int(out[2]),
+                "views_per_block": int(out[3])}
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:

@@ -222,10 +222,11 @@ constexpr int kBrickCapDefault = 4096;
 // before it flushes to device atomics.
 constexpr int kMinChunk = 4;
 
-// Host estimate of the chunk bricks of the first view at the identity pose:
-// the grid samples touched by 16 x 16 pixel tiles (every 4th tile along
-// each detector axis, at most 16 per axis) over one chunk of samples at a
-// quarter, half and three quarters of the ray.  A plan takes bricks of
+// Host estimate of the chunk bricks at the identity pose, from up to four
+// views spread over the scan (views 0, V/4, V/2, 3V/4): the grid samples
+// touched by 16 x 16 pixel tiles (every 4th tile along each detector axis,
+// at most 16 per axis) over one chunk of samples at a quarter, half and three
+// quarters of the ray.  The counts of all sampled views are pooled.  A plan takes bricks of
 // 2 * kBrickCapDefault FP64 entries (64 KiB of LDS, fewer workgroups per CU)
 // only when at least a quarter of the sampled bricks overflow
 // kBrickCapDefault yet fit the larger brick: magnified detectors over fine
@@ -244,8 +245,11 @@ int choose_brick_cap(const Params &P, const double *panels, int *slab_chunks) {
     const int step_y = std::max(4, (tiles_y + 15) / 16), step_x = std::max(4, (tiles_x + 15) / 16);
     const int sampled_chunks[3] = {chunks / 4, chunks / 2, (3 * chunks) / 4};
     int measured = 0, rescued = 0, index_over = 0, slab_measured = 0, slab_over = 0;  // over: > 2 * default
+    const int sampled_views = std::min(P.views, 4);
+    for (int sv = 0; sv < sampled_views; ++sv)
     for (int ty = 0; ty < tiles_y; ty += step_y)
         for (int tx = 0; tx < tiles_x; tx += step_x) {
+            const double *pv = panels + 12 * (size_t)((long long)sv * P.views / sampled_views);
             // the tile's rays in grid coordinates with their clipped intervals
             struct TileRay {
                 double o[3], d[3], t0, step;
@@ -256,8 +260,8 @@ int choose_brick_cap(const Params &P, const double *panels, int *slab_chunks) {
                     TileRay tr;
                     double src[3], end[3];
                     for (int a = 0; a < 3; ++a) {
-                        src[a] = panels[a] - P.go[a];
-                        end[a] = panels[3 + a] + c * panels[6 + a] + r * panels[9 + a] - panels[a];
+                        src[a] = pv[a] - P.go[a];
+                        end[a] = pv[3 + a] + c * pv[6 + a] + r * pv[9 + a] - pv[a];
                     }
                     for (int a = 0; a < 3; ++a) {
                         tr.o[a] = P.g[3 * a] * src[0] + P.g[3 * a + 1] * src[1] + P.g[3 * a + 2] * src[2];
@@ -337,8 +341,8 @@ int choose_brick_cap(const Params &P, const double *panels, int *slab_chunks) {
                     for (int c = tx * tile; c < std::min(tx * tile + tile, P.cols); ++c) {
                         double src[3], end[3], o[3], d[3];
                         for (int a = 0; a < 3; ++a) {
-                            src[a] = panels[a] - P.go[a];
-                            end[a] = panels[3 + a] + c * panels[6 + a] + r * panels[9 + a] - panels[a];
+                            src[a] = pv[a] - P.go[a];
+                            end[a] = pv[3 + a] + c * pv[6 + a] + r * pv[9 + a] - pv[a];
                         }
                         for (int a = 0; a < 3; ++a) {
                             o[a] = P.g[3 * a] * src[0] + P.g[3 * a + 1] * src[1] + P.g[3 * a + 2] * src[2];
@@ -2229,6 +2233,11 @@ int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t 
         const int forced = std::atoi(cap);
         if (forced >= 512 && forced <= 8192) P.brick_cap = forced;
     }
+    // slab chunks are a form of the FP64 volume adjoint only
+    if (const char *slab = std::getenv("DPT_AMD_SLAB_CHUNKS")) {
+        if (std::strcmp(slab, "0") == 0) P.slab_chunks = 0;
+        if (std::strcmp(slab, "1") == 0) P.slab_chunks = 1;
+    }
     p->vjp_vb = P.bundle ? 1 : choose_views_per_block(panels, views, rows, cols);
     if (const char *vb = std::getenv("DPT_AMD_VIEWS_PER_BLOCK")) {
         const int forced = std::atoi(vb);
@@ -2686,6 +2695,15 @@ int dpt_adjoint_gather_noalloc(dpt_plan *p, const double *poses, const double *c
     HIP_TRY(launch_gather(launch_params(p, poses), cotangent, rays, views, guard_word, out,
                           out_precision == DPT_ARITH_FP64, (flags & DPT_ACCUMULATE) != 0,
                           (flags & DPT_NO_CHECKS) == 0, p->d_panel_meta, p->d_panel_modes, p->image_cols, s));
+    return DPT_OK;
+}
+
+int dpt_plan_layout(const dpt_plan *p, int32_t *out) {
+    if (p == nullptr || out == nullptr) return fail(DPT_E_INVALID, "plan and out are required");
+    out[0] = p->base.brick_cap;
+    out[1] = p->base.slab_chunks;
+    out[2] = p->base.chunk_samples;
+    out[3] = p->vjp_vb;
     return DPT_OK;
 }
 

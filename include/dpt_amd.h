@@ -282,6 +282,13 @@ double dpt_kernel_ms(dpt_plan *plan, int32_t kind);
  * counterpart. */
 int64_t dpt_count_hit_rays(dpt_plan *plan, const double *poses, void *stream);
 
+/* The launch layout a plan chose: out[0] FP64 LDS brick entries of the
+ * volume adjoint, out[1] 1 when the FP64 volume adjoint chunks by slabs of
+ * grid samples instead of runs of sample indices, out[2] samples per chunk,
+ * out[3] views per workgroup of the VJP launches.  Inspection helper, no
+ * reference counterpart. */
+int dpt_plan_layout(const dpt_plan *plan, int32_t *out);
+
 const char *dpt_last_error(void);
 int32_t dpt_abi_version(void);
 

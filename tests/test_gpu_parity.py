@@ -190,6 +190,58 @@ def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch)
     assert _rel(p_gpu, p_ref) <= (1e-5 if precision == "float32" else 1e-10)
 
 
+@pytest.mark.parametrize("mode", ["both", "volume"])
+@pytest.mark.parametrize("cap,chunk", [(None, None), ("512", None), (None, "1"), ("512", "7"), (None, "64")])
+def test_fp64_slab_chunks_match_oracle(cap, chunk, mode, monkeypatch):
+    """The FP64 volume adjoint chunked by slabs of grid samples (forced on),
+    joint and volume-only, from the smallest brick (device-atomic fallback
+    chunks) up, chunk lengths 1 to 64."""
+    print(f"case slabs cap={cap} chunk={chunk} {mode}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", "1")
+    if cap is not None:
+        monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
+    if chunk is not None:
+        monkeypatch.setenv("DPT_AMD_CHUNK", chunk)
+    grid, panels, values = _cone(n=24, views=5, rows=20, cols=28, seed=6)
+    pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    ref = O.Scene(grid, 97, panels=panels)
+    gpu = NativeScene(grid, 97, panels=panels, precision="float64")
+    assert gpu.plan.layout()["slab_chunks"]
+    image, _ = ref.forward(values, pose, precision="fp64")
+    seeds = np.random.default_rng(12).uniform(-1.0, 1.0, image.size)
+    pose_grad = mode == "both"
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose, precision="fp64", pose=pose_grad)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose, pose_grad=pose_grad)
+    assert status.tolist() == [0] * len(status)
+    assert _rel(g_gpu, g_ref) <= 1e-7
+    if pose_grad:
+        assert _rel(p_gpu, p_ref) <= 1e-10
+
+
+def test_plans_choose_slab_chunks_for_a_silhouette_heavy_geometry():
+    """A 512 x 512 x 139 chest grid at 0.76 x 0.76 x 2.5 mm under a 512^2
+    panel at 2.67 mm (the reference's gather workload, benchmarks/gather.py)
+    chunks its FP64 volume adjoint by slabs with the larger brick; the
+    full-scan orbit of a 256^3 cube (360 views of 512^2, S = 1024) keeps index
+    chunks and the default brick."""
+    import bench
+
+    chest = bench.CHEST_GRID
+    grid = O.Grid(chest["shape"], chest["spacing_mm"], chest["origin_mm"], chest["orientation"])
+    ap = bench.CHEST_AP
+    panel = O.Panel(ap["source_mm"], ap["origin_mm"], ap["u"], ap["v"], ap["spacing_mm"], ap["shape"])
+    layout = NativeScene(grid, 1024, panels=[panel], precision="float64").plan.layout()
+    assert layout["slab_chunks"] and layout["brick_cap"] == 8192, layout
+    from dpt import phantoms
+
+    cube = phantoms.centred_grid((256, 256, 256))
+    geometry, _ = phantoms.circular_geometry(cube, 360, (512, 512), sod=1000.0, sdd=1500.0)
+    panels = [O.Panel(p.source_mm, p.origin_mm, p.u, p.v, p.spacing_mm, p.shape) for p in geometry.views]
+    grid = O.Grid(cube.shape, cube.spacing_mm, cube.origin_mm, cube.orientation)
+    layout = NativeScene(grid, 1024, panels=panels, precision="float64").plan.layout()
+    assert not layout["slab_chunks"] and layout["brick_cap"] == 4096, layout
+
+
 @pytest.mark.parametrize("mode", ["both", "volume", "pose"])
 @pytest.mark.parametrize("name", ["oblique", "workflow", "cone"])
 def test_fp64_vjp_matches_oracle(name, mode):
@@ -287,9 +339,13 @@ def test_half_storage_forward_and_gradients_within_budget_of_fp64(storage):
     expected, _ = ref.forward(values, precision="fp64")
     actual, status = gpu.forward(values)
     assert status.tolist() == [0] * 6
-    assert _rel(actual, expected) <= 1e-3
     seeds = np.random.default_rng(5).uniform(-1, 1, expected.size)
     g_ref, p_ref, _, _ = ref.vjp(values, seeds, precision="fp64")
     g_gpu, p_gpu, status = gpu.vjp(values, seeds.astype(np.float32))
-    assert _rel(g_gpu, g_ref) <= 1e-3
-    assert _rel(p_gpu, p_ref) <= 1e-2
+    errors = (_rel(actual, expected), _rel(g_gpu, g_ref), _rel(p_gpu, p_ref))
+    print(f"half storage {storage}: image {errors[0]:.3e} volume {errors[1]:.3e} pose {errors[2]:.3e}", flush=True)
+    # measured on MI355X: bf16 1.9e-4 / 4.2e-6 / 2.0e-3, f16 2.5e-5 / 4.2e-6 /
+    # 2.8e-4 (image / volume / pose, relative L2 against the FP64 oracle);
+    # the bars leave a factor of 2.5
+    bars = {"bfloat16": (5e-4, 1e-5, 5e-3), "float16": (6e-5, 1e-5, 7e-4)}[storage]
+    assert all(e <= b for e, b in zip(errors, bars)), (errors, bars)
