@@ -18,7 +18,11 @@ FP32 algorithm) on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import csv
+import gc
+import hashlib
 import json
+import math
 import os
 import sys
 import time
@@ -50,8 +54,19 @@ def _args():
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-rows-stride", type=int, default=1)
     parser.add_argument("--no-serving", action="store_true", help="skip the batched pose-fit serving measurement")
-    parser.add_argument("--serving-requests", type=int, default=32)
-    parser.add_argument("--serving-rate", type=float, default=10.0, help="Poisson arrival rate (requests/s)")
+    parser.add_argument("--serving-requests", type=int, default=200)
+    parser.add_argument("--serving-rates", default="10,50",
+                        help="Poisson arrival rates (requests/s) of the open-loop runs, comma separated")
+    parser.add_argument("--serving-only", action="store_true",
+                        help="only the serving measurement (every form at every rate; one JSON object)")
+    parser.add_argument("--no-guards", action="store_true", help="skip the full-scan acceptance guards")
+    parser.add_argument("--no-request", action="store_true", help="skip the single-request (chest AP) timings")
+    parser.add_argument("--record-counters", nargs="+", default=None, metavar="CSV",
+                        help="instead of the benchmark, write profiles/projector_counters.json from rocprofv3 "
+                             "counter CSVs of --kernels-once runs (copy-calibration CSVs first, see "
+                             "--calibration-bytes)")
+    parser.add_argument("--calibration-bytes", type=float, default=0.0,
+                        help="bytes read and written by the copy kernel of the calibration CSVs")
     parser.add_argument("--no-transport", action="store_true", help="skip the Monte Carlo transport measurement")
     parser.add_argument("--no-precision", action="store_true", help="skip the BF16/FP16 storage error report")
     parser.add_argument("--no-gather", action="store_true", help="skip the deterministic adjoint gather measurement")
@@ -78,6 +93,68 @@ def _args():
         help="instead of the benchmark, run a plain device copy of this many GiB (counter calibration)",
     )
     return parser.parse_args()
+
+
+COUNTERS = ROOT / "profiles" / "projector_counters.json"
+
+
+def sources_digest() -> str:
+    """SHA-256 over the HIP sources and the C-ABI header: the code a counter
+    profile measured."""
+    h = hashlib.sha256()
+    paths = sorted(p for p in (ROOT / "dpt" / "csrc").iterdir() if p.suffix in (".hip", ".h"))
+    for path in [*paths, ROOT / "include" / "dpt_amd.h"]:
+        h.update(path.name.encode())
+        h.update(path.read_bytes())
+    return h.hexdigest()
+
+
+def _counter_rows(path):
+    """{(dispatch, kernel): {counter: value}} of one rocprofv3 counter CSV."""
+    out = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            key = (int(row["Dispatch_Id"]), row["Kernel_Name"])
+            out.setdefault(key, {})[row["Counter_Name"]] = float(row["Counter_Value"])
+    return out
+
+
+def record_counters(paths, calibration_bytes: float) -> int:
+    """Write profiles/projector_counters.json from rocprofv3 counter CSVs.
+
+    CSVs whose kernels include no projector kernel are the copy calibration:
+    the largest kernel there moves ``calibration_bytes`` in and out, which
+    scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) to bytes.  Projector
+    kernels (forward32 / vjp32) keep their last dispatch per name; counters of
+    several passes over the same driver are merged by kernel name."""
+    fetch_scale = write_scale = 1024.0
+    kernels = {}
+    for path in paths:
+        rows = _counter_rows(path)
+        projector = {k: v for k, v in rows.items() if "forward32" in k[1] or "vjp32" in k[1]}
+        if not projector:
+            copy = max(rows.values(), key=lambda c: c.get("FETCH_SIZE", 0.0) + c.get("WRITE_SIZE", 0.0))
+            if calibration_bytes > 0 and copy.get("FETCH_SIZE"):
+                fetch_scale = calibration_bytes / copy["FETCH_SIZE"]
+            if calibration_bytes > 0 and copy.get("WRITE_SIZE"):
+                write_scale = calibration_bytes / copy["WRITE_SIZE"]
+            continue
+        for (_, name), values in sorted(projector.items()):
+            kernels.setdefault(name, {}).update(values)
+    for values in kernels.values():
+        if "FETCH_SIZE" in values:
+            values["fetch_bytes"] = values["FETCH_SIZE"] * fetch_scale
+        if "WRITE_SIZE" in values:
+            values["write_bytes"] = values["WRITE_SIZE"] * write_scale
+    record = {
+        "sources_sha256": sources_digest(),
+        "fetch_bytes_per_kib_counted": fetch_scale,
+        "write_bytes_per_kib_counted": write_scale,
+        "kernels": kernels,
+    }
+    COUNTERS.write_text(json.dumps(record, indent=1, sort_keys=True) + "\n")
+    print(json.dumps({"written": str(COUNTERS), "kernels": sorted(kernels)}), flush=True)
+    return 0
 
 
 def copy_calibration(gib: float) -> int:
@@ -180,10 +257,22 @@ def serving_workload(count: int, seed: int = 404):
     return volumes, requests, truths
 
 
-def serving_benchmark(count: int, rate: float) -> dict:
-    """Batched BF16 service (continuous batching, hipGraph steps) versus the
-    plain per-request FP32 form (one solve at a time, direct launches), both
-    closed-loop (all requests queued) and open-loop (Poisson arrivals)."""
+SERVING_FORMS = {
+    # name: (storage, layout, slots, hipGraph steps)
+    "batched_bf16": ("bfloat16", "cells", 64, True),
+    # ablation: the batched engine with the FP32 field in the cell layout
+    "batched_fp32": ("float32", "cells", 64, True),
+    # the reference's form: FP32 linear field, one solve at a time, direct launches
+    "plain_fp32": ("float32", "linear", 1, False),
+}
+
+
+def serving_benchmark(count: int, rates, forms=("batched_bf16", "batched_fp32", "plain_fp32"),
+                      open_forms=("batched_bf16",)) -> dict:
+    """Pose-fit service: each form closed-loop (all requests queued at t = 0)
+    and, for ``open_forms``, open-loop at each Poisson arrival rate in
+    ``rates``; ``count`` requests per form and per rate.  Latency is arrival
+    to completion; p50 / p99 over all requests of a run."""
     import dataclasses
 
     import numpy as np
@@ -191,52 +280,116 @@ def serving_benchmark(count: int, rate: float) -> dict:
     from dpt import serving
 
     volumes, requests, truths = serving_workload(count)
-    arrivals = serving.poisson_arrivals(count, rate, seed=405)
-    timed = [dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, arrivals)]
-    out = {"requests": count, "arrival_rate_per_s": rate}
-    # the served form keeps volumes resident in BF16 cell layout; the plain
-    # form gathers FP32 from the linear array, as the reference does
-    forms = {"batched_bf16": ("bfloat16", "cells", 64, True), "plain_fp32": ("float32", "linear", 1, False)}
-    for name, (storage, layout, slots, graph) in forms.items():
+    out = {"requests_per_run": count, "arrival_rates_per_s": list(rates), "forms": {}}
+    for name in forms:
+        storage, layout, slots, graph = SERVING_FORMS[name]
         registry = serving.VolumeRegistry()
         for key, grid, values in volumes:
             registry.add(key, values, grid, storage=storage, layout=layout)
         engine = serving.PoseFitEngine(registry, max_requests=slots, max_tiles=1 << 19, use_graph=graph)
         engine.run(requests[:2], realtime=False)  # warm-up: code objects, graphs
         closed = engine.run(requests, realtime=False)
-        opened = engine.run(timed, realtime=True)
-        sweep = {}
-        if name == "batched_bf16":
-            # latency across the arrival rates of the served workload (5-200 req/s)
-            for r in (5.0, 50.0, 200.0):
-                at = serving.poisson_arrivals(count, r, seed=406)
+        errors = np.asarray([serving.pose_error(r.pose, truths[int(r.key[3:])]) for r in closed.results])
+        row = {"storage": storage, "layout": layout, "slots": slots, "graph": graph,
+               "closed_loop": closed.summary(),
+               "median_rotation_error_deg": round(float(np.median(errors[:, 0])), 5),
+               "p99_rotation_error_deg": round(float(np.percentile(errors[:, 0], 99)), 5),
+               "median_translation_error_mm": round(float(np.median(errors[:, 1])), 5)}
+        _log(f"serving {name}: {row['closed_loop']['solves_per_s']} solves/s closed-loop")
+        if name in open_forms:
+            row["open_loop"] = {}
+            for k, rate in enumerate(rates):
+                at = serving.poisson_arrivals(count, rate, seed=405 + k)
                 run = engine.run([dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, at)],
                                  realtime=True)
-                sweep[f"{r:g}_per_s"] = {k: run.summary()[k] for k in ("solves_per_s", "p50_latency_ms",
-                                                                       "p99_latency_ms", "peak_active")}
-        errors = []
-        for result in closed.results:
-            k = int(result.key[3:])
-            errors.append(serving.pose_error(result.pose, truths[k]))
-        errors = np.asarray(errors)
-        out[name] = {
-            "storage": storage,
-            "layout": layout,
-            "slots": slots,
-            "graph": graph,
-            "closed_loop": closed.summary(),
-            "open_loop": opened.summary(),
-            **({"open_loop_rates": sweep} if sweep else {}),
-            "median_rotation_error_deg": round(float(np.median(errors[:, 0])), 5),
-            "median_translation_error_mm": round(float(np.median(errors[:, 1])), 5),
-        }
+                row["open_loop"][f"{rate:g}_per_s"] = run.summary()
+                _log(f"serving {name} at {rate:g} req/s: p50/p99 {run.summary()['p50_latency_ms']}/"
+                     f"{run.summary()['p99_latency_ms']} ms")
         engine.close()
-        _log(f"serving {name}: {out[name]['closed_loop']['solves_per_s']} solves/s closed-loop, "
-             f"p50/p99 {out[name]['open_loop']['p50_latency_ms']}/{out[name]['open_loop']['p99_latency_ms']} ms")
-    out["throughput_ratio_closed_loop"] = round(
-        out["batched_bf16"]["closed_loop"]["solves_per_s"] / out["plain_fp32"]["closed_loop"]["solves_per_s"], 3
-    )
+        out["forms"][name] = row
+    if "batched_bf16" in out["forms"] and "plain_fp32" in out["forms"]:
+        out["throughput_ratio_closed_loop"] = round(
+            out["forms"]["batched_bf16"]["closed_loop"]["solves_per_s"]
+            / out["forms"]["plain_fp32"]["closed_loop"]["solves_per_s"], 3)
     return out
+
+
+# The reference's request-sized case (docs/performance.md, "Completed
+# operation timings": chest AP, scalar, central cotangent, 512^2, 1024
+# samples): FP32 volume VJP 0.958784 ms and pose VJP 0.580912 ms (CUDA
+# events), forward + volume/pose VJP 3.834406 / 4.161676 ms median / p95
+# completed wall; FP64 3.624160, 8.693632 and 20.512820 / 21.108925 ms.
+REQUEST_REFERENCE_MS = {
+    "fp32": {"volume_vjp": 0.958784, "pose_vjp": 0.580912, "iteration_p50": 3.834406, "iteration_p95": 4.161676},
+    "fp64": {"volume_vjp": 3.624160, "pose_vjp": 8.693632, "iteration_p50": 20.512820, "iteration_p95": 21.108925},
+}
+
+
+def request_benchmark(repeats: int = 300) -> dict:
+    """One request of the reference's shape: the chest grid (512 x 512 x 139
+    at 0.762 x 0.762 x 2.5 mm; synthetic chest phantom content), its AP view on
+    a 512^2 panel, S = 1024, the central-window cotangent (unit sum).  Volume
+    and pose VJPs alone: torch events around autograd.grad on a retained
+    graph; the iteration: forward + volume and pose VJP, completed wall time;
+    medians and p95 over ``repeats``."""
+    import numpy as np
+    import torch
+
+    from dpt import phantoms
+    from dpt.geometry import DetectorGeometry
+    from dpt.torch import Projector
+    from dpt.volumes import GridSpec
+
+    device = torch.device("cuda:0")
+    grid = GridSpec(**CHEST_GRID)
+    geometry = DetectorGeometry(**CHEST_AP)
+    values = phantoms.chest_phantom(grid, seed=2027)
+    rows, cols = CHEST_AP["shape"]
+    result = {"workload": "chest AP view, 512 x 512 x 139 grid (synthetic content), 512^2 panel, S = 1024, "
+                          "central-window cotangent", "repeats": repeats}
+    for precision in ("fp32", "fp64"):
+        module = Projector(grid=grid, geometry=geometry, precision=precision, samples_per_ray=1024)
+        dtype = torch.float32 if precision == "fp32" else torch.float64
+        seed = torch.zeros((1, rows, cols), dtype=dtype, device=device)
+        seed[:, rows * 5 // 16: rows * 11 // 16, cols * 5 // 16: cols * 11 // 16] = 1.0
+        seed /= seed.sum()
+        field = torch.tensor(values, device=device, requires_grad=True)
+        pose = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
+        timings = {"volume_vjp": [], "pose_vjp": [], "iteration": []}
+        for _ in range(3):  # warm-up
+            torch.autograd.grad(module(field, pose), (field, pose), seed)
+        torch.cuda.synchronize()
+        for name, inputs in (("volume_vjp", (field,)), ("pose_vjp", (pose,))):
+            image = module(field if name == "volume_vjp" else field.detach(),
+                           pose if name == "pose_vjp" else pose.detach())
+            for _ in range(repeats):
+                start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                start.record()
+                torch.autograd.grad(image, inputs, seed, retain_graph=True)
+                stop.record()
+                stop.synchronize()
+                timings[name].append(start.elapsed_time(stop))
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            torch.autograd.grad(module(field, pose), (field, pose), seed)
+            torch.cuda.synchronize()
+            timings["iteration"].append((time.perf_counter() - t0) * 1e3)
+        ref = REQUEST_REFERENCE_MS[precision]
+        row = {
+            "volume_vjp_ms": round(float(np.median(timings["volume_vjp"])), 4),
+            "pose_vjp_ms": round(float(np.median(timings["pose_vjp"])), 4),
+            "iteration_p50_ms": round(float(np.median(timings["iteration"])), 4),
+            "iteration_p95_ms": round(float(np.percentile(timings["iteration"], 95)), 4),
+            "reference_ms": ref,
+        }
+        row["iteration_speedup_vs_reference"] = round(ref["iteration_p50"] / row["iteration_p50_ms"], 3)
+        result[precision] = row
+        _log(f"request {precision}: volume VJP {row['volume_vjp_ms']} ms, pose VJP {row['pose_vjp_ms']} ms, "
+             f"iteration p50/p95 {row['iteration_p50_ms']}/{row['iteration_p95_ms']} ms "
+             f"(reference {ref['iteration_p50']}/{ref['iteration_p95']})")
+        del module, field, pose
+    return result
 
 
 def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
@@ -284,8 +437,26 @@ def transport_benchmark(histories: int, cpu_histories: int, cpu: bool) -> dict:
 
             model = T.Model(*bind_model(spec, material_ids=labels, absorption=a, scattering=s, physical=physical))
             t0 = time.perf_counter()
-            T.trace(model, dens, source, 2026, 0, cpu_histories, records=False)
+            cpu = T.trace(model, dens, source, 2026, 0, cpu_histories, records=False)
             elapsed = time.perf_counter() - t0
+            # the same histories on the GPU: tallies and statuses against the oracle's
+            if cpu_histories <= histories:
+                import numpy as np
+
+                same = transport.simulate(dens, source=source, batch=transport.HistoryBatch(2026, 0, cpu_histories),
+                                          workspace=sim)
+                mean, _ = T.moments(cpu["tally"], cpu_histories)
+                checks = {}
+                for k, image in enumerate((same.mean, same.primary_mean, same.scatter_mean)):
+                    g = image.cpu().numpy().astype(np.float64).ravel()
+                    c = mean[k].ravel()
+                    checks[("total", "primary", "scatter")[k]] = float(np.linalg.norm(g - c) / np.linalg.norm(c))
+                out[name]["tallies_vs_oracle"] = {
+                    "histories": cpu_histories,
+                    "relative_l2": checks,
+                    "status_histogram_equal": list(same.status_histogram.counts) == cpu["histogram"].tolist(),
+                    "passed": max(checks.values()) <= 1e-4,
+                }
             out[name]["cpu_baseline"] = {
                 "value": round(cpu_histories / elapsed, 1),
                 "unit": "histories/s",
@@ -561,6 +732,244 @@ def cell_gauss_benchmark(grid, geometry, values, target, samples: int) -> dict:
     return result
 
 
+# The reference's full-scan acceptance guards (benchmarks/config/fullscan.json
+# "quality", benchmarks/config/fullscan_guards.json, benchmarks/fullscan_guards.py):
+# thresholds, views, pixels, tangents, directions and steps as published there.
+GUARD = {
+    "image_fp32_vs_fp64": {"max_abs": 5e-4, "rmse": 2e-5, "relative_l2": 1e-5},
+    "volume_adjoint_max": 2e-5,
+    "views": (0, 37, 113),
+    "pixels": ((127, 193), (187, 251), (256, 256), (331, 301), (401, 278)),
+    "anchor_tangent": (0.137, -0.219, 0.083, 0.0017, -0.0023, 0.0009),
+    "pose_direction": (0.31, -0.27, 0.19, 0.0007, -0.0011, 0.0009),
+    "pose_fd": {"fp32": {"rtol": 0.02, "atol": 2e-5, "steps": (0.02, 0.04, 0.01)},
+                "fp64": {"rtol": 0.005, "atol": 1e-7, "steps": (1e-6, 1e-5, 1e-7)}},
+    "volume_fd": {"fp32": {"rtol": 0.005, "atol": 1e-6}, "fp64": {"rtol": 2e-5, "atol": 1e-8}},
+    "volume_step": 0.01,
+    "ray_tolerance": {"fp32": 5e-4, "fp64": 2e-10},
+}
+
+
+def guard_cotangent(views: int, rows: int, cols: int):
+    """The full scan's dense objective (fullscan.json "cotangent"): raw = 1 +
+    0.20 sin(theta) + 0.10 cos(2 theta) + 0.15 r + 0.10 c + 0.05 sin(theta) c,
+    theta = 2 pi k / views, r and c pixel centres in [-1, 1]; divided by the
+    global L1 sum in FP64, then cast once to FP32."""
+    import numpy as np
+
+    theta = 2.0 * np.pi * np.arange(views)[:, None, None] / views
+    r = (2.0 * (np.arange(rows) + 0.5) / rows - 1.0)[None, :, None]
+    c = (2.0 * (np.arange(cols) + 0.5) / cols - 1.0)[None, None, :]
+    raw = 1.0 + 0.20 * np.sin(theta) + 0.10 * np.cos(2 * theta) + 0.15 * r + 0.10 * c + 0.05 * np.sin(theta) * c
+    return (raw / np.abs(raw).sum()).astype(np.float32)
+
+
+def independent_ray(values, grid, panel, pose, row, column, samples) -> float:
+    """One ray on the CPU, written apart from the product path: FP64 world ->
+    object -> grid transform, slab clip of the segment to the voxel-face
+    support, ``samples`` midpoints, explicit eight-corner trilinear weights
+    with clamped indices (the algorithm of benchmarks/fullscan_guards.py:15-53)."""
+    import numpy as np
+
+    source = np.asarray(panel.source_mm, dtype=np.float64)
+    target = (np.asarray(panel.origin_mm) + column * panel.spacing_mm[0] * np.asarray(panel.u)
+              + row * panel.spacing_mm[1] * np.asarray(panel.v))
+    rotation = np.asarray(pose.rotation, dtype=np.float64).reshape(3, 3)
+    axes = np.asarray(grid.orientation, dtype=np.float64).reshape(3, 3)
+    ends = (np.stack((source, target)) - np.asarray(pose.translation_mm)) @ rotation
+    ends = ((ends - np.asarray(grid.origin_mm)) @ axes) / np.asarray(grid.spacing_mm)
+    start, delta = ends[0], ends[1] - ends[0]
+    shape = np.asarray(values.shape[::-1])
+    enter, leave = 0.0, 1.0
+    for axis in range(3):
+        low, high = -0.5, shape[axis] - 0.5
+        if delta[axis] == 0:
+            if not low <= start[axis] <= high:
+                return 0.0
+            continue
+        a, b = sorted(((low - start[axis]) / delta[axis], (high - start[axis]) / delta[axis]))
+        enter, leave = max(enter, a), min(leave, b)
+    if leave <= enter:
+        return 0.0
+    t = enter + (np.arange(samples) + 0.5) * (leave - enter) / samples
+    points = np.clip(start + t[:, None] * delta, 0, shape - 1)
+    lower = np.floor(points).astype(int)
+    upper = np.minimum(lower + 1, shape - 1)
+    frac = points - lower
+    total = np.zeros(samples)
+    for bits in np.ndindex(2, 2, 2):
+        b = np.asarray(bits[::-1])  # (x, y, z) corner bits
+        idx = np.where(b, upper, lower)
+        weight = np.prod(np.where(b, frac, 1.0 - frac), axis=1)
+        total += weight * values[idx[:, 2], idx[:, 1], idx[:, 0]]
+    return float(total.mean() * np.linalg.norm(target - source) * (leave - enter))
+
+
+def fullscan_guards(grid, geometry, values, samples: int) -> dict:
+    """The reference's acceptance guards on the full-scan workload: the FP32
+    image against the FP64 image of every view, the volume adjoint identity
+    <A mu, c> = <mu, A^T c> over every view with the dense objective, central
+    pose and volume differences at the published steps (guard views, anchor
+    pose, FP32 and FP64), and independent CPU rays on the guard pixels.  Each
+    check records pass / fail against the published thresholds."""
+    import numpy as np
+    import torch
+
+    from dpt.geometry import BatchedGeometry, RigidTransform, compose_pose
+    from dpt.torch import Projector
+
+    device = torch.device("cuda:0")
+    views, (rows, cols) = len(geometry.views), geometry.shape_rows_columns
+    out = {}
+    field = torch.tensor(values, device=device, requires_grad=True)
+    with torch.no_grad():
+        image32 = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)(field.detach())
+        image64 = Projector(grid=grid, geometry=geometry, precision="fp64", samples_per_ray=samples)(field.detach())
+    diff = image32.double() - image64
+    bar = GUARD["image_fp32_vs_fp64"]
+    check = {"max_abs": float(diff.abs().max()), "rmse": float(diff.square().mean().sqrt()),
+             "relative_l2": float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(image64))}
+    check["passed"] = all(check[k] <= bar[k] for k in bar)
+    out["image_fp32_vs_fp64"] = {**check, "limits": bar}
+    del image64, diff
+    dense = guard_cotangent(views, rows, cols)
+    seed = torch.tensor(dense, device=device)
+    module = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)
+    image = module(field)
+    (field_vjp,) = torch.autograd.grad(image, field, seed)
+    lhs = float((image.double() * seed.double()).sum())
+    rhs = float((field.detach().double() * field_vjp.double()).sum())
+    discrepancy = abs(lhs - rhs) / max(abs(lhs), abs(rhs), 1e-300)
+    out["volume_adjoint"] = {"lhs": lhs, "rhs": rhs, "relative_discrepancy": discrepancy,
+                             "limit": GUARD["volume_adjoint_max"], "passed": discrepancy <= GUARD["volume_adjoint_max"]}
+    del image, field_vjp
+    picks = GUARD["views"]
+    subset = BatchedGeometry(tuple(geometry.views[k] for k in picks))
+    anchor = compose_pose(RigidTransform(), GUARD["anchor_tangent"])
+    seed_host = dense[list(picks)].astype(np.float64)
+    seed_host /= np.abs(seed_host).sum()
+    direction = torch.tensor(GUARD["pose_direction"], dtype=torch.float64, device=device)
+    z, y, x = np.ogrid[:grid.shape[0], :grid.shape[1], :grid.shape[2]]
+    base = values.astype(np.float64)
+    step = GUARD["volume_step"]
+    perturbation = base * 0.01 + (base > 0) * 1e-5 * (1 + 0.2 * np.cos(x / 13) + 0.2 * np.cos(y / 17)
+                                                       + 0.2 * np.cos(z / 19))
+    perturbation = np.minimum(perturbation, 0.5 * base / step)
+    plus_host = (base + step * perturbation).astype(np.float32)
+    minus_host = (base - step * perturbation).astype(np.float32)
+    realised = (plus_host.astype(np.float64) - minus_host.astype(np.float64)) / (2 * step)
+    for precision in ("fp32", "fp64"):
+        projector = Projector(grid=grid, geometry=subset, anchor=anchor, precision=precision,
+                              samples_per_ray=samples)
+        dtype = torch.float32 if precision == "fp32" else torch.float64
+        weights = torch.tensor(seed_host, dtype=dtype, device=device)
+        tangent = torch.zeros(6, dtype=torch.float64, device=device, requires_grad=True)
+        image = projector(field, tangent)
+        volume_grad, pose_grad = torch.autograd.grad(image, (field, tangent), weights)
+
+        def objective(output):
+            return float((output.double() * weights.double()).sum())
+
+        limits = GUARD["pose_fd"][precision]
+        analytic = float((pose_grad * direction).sum())
+        checks = []
+        with torch.no_grad():
+            for h in limits["steps"]:
+                fd = (objective(projector(field.detach(), h * direction))
+                      - objective(projector(field.detach(), -h * direction))) / (2 * h)
+                error = abs(analytic - fd)
+                allowed = limits["atol"] + limits["rtol"] * abs(analytic)
+                checks.append({"step": h, "analytic": analytic, "finite_difference": fd, "absolute_error": error,
+                               "allowed_absolute_error": allowed, "passed": error <= allowed,
+                               "acceptance_step": h == limits["steps"][0]})
+        out[f"pose_fd_{precision}"] = {"checks": checks, "passed": checks[0]["passed"]}
+        bound = GUARD["volume_fd"][precision]
+        with torch.no_grad():
+            plus = objective(projector(torch.tensor(plus_host, device=device), tangent.detach()))
+            minus = objective(projector(torch.tensor(minus_host, device=device), tangent.detach()))
+        analytic_v = float(np.sum(volume_grad.cpu().numpy().astype(np.float64) * realised))
+        fd_v = (plus - minus) / (2 * step)
+        error = abs(analytic_v - fd_v)
+        allowed = bound["atol"] + bound["rtol"] * abs(analytic_v)
+        out[f"volume_fd_{precision}"] = {"analytic": analytic_v, "finite_difference": fd_v, "absolute_error": error,
+                                         "allowed_absolute_error": allowed, "passed": error <= allowed, "step": step}
+        actual = image.detach().cpu().numpy()
+        tolerance = GUARD["ray_tolerance"][precision]
+        rays = []
+        for local, view in enumerate(picks):
+            for row, column in GUARD["pixels"]:
+                expected = independent_ray(values, grid, subset.views[local], anchor, row, column, samples)
+                observed = float(actual[local, row, column])
+                rays.append({"view": view, "pixel": [row, column], "reference": expected, "actual": observed,
+                             "absolute_error": abs(observed - expected), "passed": abs(observed - expected) <= tolerance})
+        out[f"independent_rays_{precision}"] = {"tolerance": tolerance, "rays": rays,
+                                               "max_absolute_error": max(r["absolute_error"] for r in rays),
+                                               "passed": all(r["passed"] for r in rays)}
+        del projector, image, volume_grad, pose_grad
+    out["passed"] = all(v["passed"] for v in out.values() if isinstance(v, dict) and "passed" in v)
+    _log("guards: " + ", ".join(f"{k} {'pass' if v['passed'] else 'FAIL'}" for k, v in out.items()
+                                if isinstance(v, dict)))
+    return out
+
+
+def roofline_report(args, fwd_ms: float, vjp_ms: float, effective: int, half: bool) -> dict:
+    """Roofline of the dominant kernel (the joint VJP at the benchmark's
+    shape).  ``achieved`` is its measured HBM traffic per launch (rocprofv3
+    FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel, recorded in
+    profiles/projector_counters.json for the sources digest it was measured
+    at) over its launch time (HIP events on its launch stream in the timed
+    region); no figure is given when the recorded digest is not the benched
+    sources'.  The gathers and scatters' algorithmic bytes (64 B per effective
+    sample, mostly served from L1/L2/LDS) are reported as a rate beside it,
+    not as a fraction.  The kernel's binding resource is instruction issue:
+    ``valu`` sets its VALU instruction rate (counted instructions over launch
+    time) against the issue rate of independent v_fma_f32 / v_add_u32 chains
+    measured on this GPU in this run."""
+    import torch
+
+    from dpt import _native
+
+    gather_b = 16 if half else GATHER_B
+    if vjp_ms >= fwd_ms:
+        kernel, ms, per_sample = "vjp32_kernel<float, true, true, 12, false, 2>", vjp_ms, gather_b + SCATTER_B
+    else:
+        kernel, ms, per_sample = "forward32_kernel<float, false, 2>", fwd_ms, gather_b
+    digest = sources_digest()
+    recorded, counters = None, None
+    if COUNTERS.exists():
+        recorded = json.loads(COUNTERS.read_text())
+        if recorded.get("sources_sha256") == digest and not half and args.samples == SAMPLES and \
+                args.views == VIEWS:
+            counters = next((v for k, v in recorded["kernels"].items() if kernel in k), None)
+    seconds = ms * 1e-3
+    report = {"bound": "hbm", "kernel": kernel, "kernel_ms": round(ms, 3), "forward_kernel_ms": round(fwd_ms, 3),
+              "vjp_kernel_ms": round(vjp_ms, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "achieved": None, "frac": None, "traffic": None,
+              "counters": {"profile": str(COUNTERS.relative_to(ROOT)), "sources_sha256": digest,
+                           "recorded_sha256": recorded.get("sources_sha256") if recorded else None,
+                           "matches": counters is not None}}
+    if counters is not None and "fetch_bytes" in counters and "write_bytes" in counters:
+        traffic = counters["fetch_bytes"] + counters["write_bytes"]
+        report["traffic"] = round(traffic)
+        report["achieved"] = round(traffic / seconds / 1e9, 1)
+        report["frac"] = round(traffic / seconds / 1e9 / HBM_PEAK_GBS, 4)
+    report["algorithmic_bytes_per_effective_sample"] = per_sample
+    report["algorithmic_rate_gbs"] = round(effective * per_sample / seconds / 1e9, 1)
+    scratch = torch.empty(2048 * 256, dtype=torch.float32, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    fma = _native.valu_probe(0, scratch.data_ptr(), stream)
+    add = _native.valu_probe(1, scratch.data_ptr(), stream)
+    valu = {"unit": "wave64 VALU instructions/s", "peak": round(max(fma, add), 1),
+            "probe_v_fma_f32": round(fma, 1), "probe_v_add_u32": round(add, 1), "achieved": None, "frac": None}
+    if counters is not None and "SQ_INSTS_VALU" in counters:
+        rate = counters["SQ_INSTS_VALU"] / seconds
+        valu["achieved"] = round(rate, 1)
+        valu["frac"] = round(rate / valu["peak"], 4)
+        valu["lane_instructions_per_effective_sample"] = round(counters["SQ_INSTS_VALU"] * 64 / effective, 1)
+    report["valu"] = valu
+    return report
+
+
 def _log(message: str) -> None:
     print(f"[bench] {message}", file=sys.stderr, flush=True)
 
@@ -704,8 +1113,15 @@ def main() -> int:
             flush=True,
         )
         return 2
+    if args.record_counters:
+        return record_counters(args.record_counters, args.calibration_bytes)
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
+    if args.serving_only:
+        rates = [float(r) for r in args.serving_rates.split(",") if r]
+        print(json.dumps(serving_benchmark(args.serving_requests, rates, open_forms=("batched_bf16", "plain_fp32"))),
+              flush=True)
+        return 0
     if args.kernels_once:
         return kernels_once(args)
     if args.transport_only:
@@ -770,6 +1186,9 @@ def main() -> int:
         _log(f"warmup {i + 1}/{args.warmup} done")
 
     times, fwd_ms, vjp_ms = [], [], []
+    # Python's cyclic collector runs between steps, not inside them
+    gc.collect()
+    gc.disable()
     torch.cuda.synchronize()
     start = time.perf_counter()
     for i in range(args.steps):
@@ -781,6 +1200,7 @@ def main() -> int:
         vjp_ms.append(entry_fwd.plan.kernel_ms(1))
         _log(f"step {i + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms (forward kernel {fwd_ms[-1]:.1f} ms, vjp kernel {vjp_ms[-1]:.1f} ms)")
     total = time.perf_counter() - start
+    gc.enable()
     assert all(torch.isfinite(g).all() for g in grads)
 
     # the reference's component table (docs/performance/fullscan.md:248-255,
@@ -829,20 +1249,7 @@ def main() -> int:
     fwd = float(np.mean(fwd_ms))
     vjp = float(np.mean(vjp_ms))
     half = args.storage != "float32"
-    gather_b = 16 if half else GATHER_B
-    if vjp >= fwd:
-        kernel, ms, per_sample = "vjp32_kernel (joint volume + pose)", vjp, gather_b + SCATTER_B
-    else:
-        kernel, ms, per_sample = "forward32_kernel", fwd, gather_b
-    achieved = effective * per_sample / (ms * 1e-3) / 1e9
-    traffic = None
-    traffic_file = ROOT / "profiles" / "traffic.json"
-    if traffic_file.exists():
-        try:
-            traffic = json.loads(traffic_file.read_text()).get(kernel.split()[0])
-        except (ValueError, OSError):
-            traffic = None
-    iteration_bytes = effective * (2 * gather_b + SCATTER_B)
+    roofline = roofline_report(args, fwd, vjp, effective, half)
     result = {
         "metric": "Gsamples/s fwd+vol+pose VJP 256^3/360v/512^2; p50/p99 batched-solve latency",
         "value": round(gsamples, 3),
@@ -872,30 +1279,24 @@ def main() -> int:
             "storage": args.storage,
             "gather": "cells" if gather_cells(decode(make()._configuration)) else "linear",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kernel,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_effective_sample": per_sample,
-            "kernel_ms": round(ms, 3),
-            "forward_kernel_ms": round(fwd, 3),
-            "vjp_kernel_ms": round(vjp, 3),
-            "iteration_frac": round(iteration_bytes / median / 1e9 / HBM_PEAK_GBS, 4),
-            "note": "algorithmic bytes over kernel time; frac > 1 means the gathers are served from L1/L2 "
-                    "(traffic is the calibrated L2-to-fabric bytes); the kernel is bound by VALU issue",
-        },
+        "roofline": roofline,
         "reference_blackwell_iteration_gsamples": 65.5,
         "components": components,
     }
+    if not args.no_guards:
+        _log("full-scan acceptance guards")
+        result["guards"] = fullscan_guards(grid, geometry, values, args.samples)
+    if not args.no_request:
+        _log("single request (chest AP)")
+        result["request"] = request_benchmark()
     if not args.no_serving:
         _log("serving benchmark")
-        result["serving"] = serving_benchmark(args.serving_requests, args.serving_rate)
-        result["p50_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p50_latency_ms"]
-        result["p99_ms"] = result["serving"]["batched_bf16"]["open_loop"]["p99_latency_ms"]
+        rates = [float(r) for r in args.serving_rates.split(",") if r]
+        result["serving"] = serving_benchmark(args.serving_requests, rates)
+        served = result["serving"]["forms"]["batched_bf16"]["open_loop"][f"{rates[0]:g}_per_s"]
+        result["p50_ms"] = served["p50_latency_ms"]
+        result["p99_ms"] = served["p99_latency_ms"]
+        result["latency_rate_per_s"] = rates[0]
     if not args.no_precision:
         _log("precision benchmark")
         result["precision"] = precision_benchmark(grid, geometry, values, target)
@@ -913,8 +1314,18 @@ def main() -> int:
     if not args.no_cpu_baseline:
         _log("timing CPU oracle baseline")
         result["cpu_baseline"] = cpu_baseline(grid, geometry, values, args.samples, args.cpu_rows_stride)
+    result["children_at_exit"] = _children()
     print(json.dumps(result), flush=True)
     return 0
+
+
+def _children() -> list:
+    """Processes this run started that are still alive (expected: none)."""
+    try:
+        import psutil
+    except ImportError:
+        return []
+    return [" ".join(c.cmdline()[:4]) or c.name() for c in psutil.Process().children(recursive=True)]
 
 
 if __name__ == "__main__":

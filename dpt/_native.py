@@ -37,6 +37,7 @@ EXPORTS = (
     "dpt_kernel_ms",
     "dpt_count_hit_rays",
     "dpt_plan_layout",
+    "dpt_valu_probe",
     "dpt_last_error",
     "dpt_abi_version",
     "dpt_fit_create",
@@ -214,6 +215,8 @@ def lib():
         h.dpt_count_hit_rays.argtypes = [vp, vp, vp]
         h.dpt_plan_layout.argtypes = [vp, vp]
         h.dpt_plan_layout.restype = ctypes.c_int
+        h.dpt_valu_probe.argtypes = [i32, vp, ctypes.POINTER(ctypes.c_double), vp]
+        h.dpt_valu_probe.restype = ctypes.c_int
         h.dpt_count_hit_rays.restype = i64
         h.dpt_last_error.argtypes = []
         h.dpt_last_error.restype = ctypes.c_char_p
@@ -384,6 +387,15 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def valu_probe(kind: int, scratch: int, stream: int) -> float:
+    """Wave64 VALU instructions per second of 16 independent v_fma_f32
+    (kind 0) or v_add_u32 (kind 1) chains per lane at 8 waves per SIMD;
+    scratch: device buffer of 2048 * 256 floats."""
+    rate = ctypes.c_double()
+    check(lib().dpt_valu_probe(kind, scratch, ctypes.byref(rate), stream), "dpt_valu_probe")
+    return float(rate.value)
 
 
 def cell_count(shape) -> int:

@@ -28,6 +28,7 @@ SOURCES = [
     PACKAGE / "csrc" / "gather.hip",
     PACKAGE / "csrc" / "fit.hip",
     PACKAGE / "csrc" / "transport.hip",
+    PACKAGE / "csrc" / "probe.hip",
 ]
 OUTPUT = PACKAGE / "_lib" / "libdpt_amd.so"
 ARCH = os.environ.get("DPT_AMD_ARCH", "gfx950")

@@ -177,16 +177,33 @@ def _check_extent(decoded: Decoded) -> None:
 # Largest cell-layout copy (bytes) a plan keeps by default; larger fields
 # gather from the linear array.
 CELL_GATHER_MAX_BYTES = 32 << 30
+# The copy is rebuilt at every call (about 36 bytes of traffic per cell) and
+# saves under a picosecond per sample (full scan: 48 ms over 6.2e10 samples),
+# so a plan takes it only when a call samples at least this many times per
+# cell: the 360-view full scan does (~5800), one 512^2 view of a 512 x 512 x
+# 139 chest grid does not (~7.5).
+CELL_GATHER_MIN_SAMPLES_PER_CELL = 64
 
 
 def gather_cells(decoded: Decoded) -> bool:
     """FP32 sampling gathers corners from a cell-layout copy of the field
-    (one or two 16-byte loads per cell instead of four 8-byte loads) unless
-    DPT_AMD_GATHER=linear or the copy would exceed CELL_GATHER_MAX_BYTES."""
-    if decoded.spec.precision != "float32" or os.environ.get("DPT_AMD_GATHER") == "linear":
+    (one or two 16-byte loads per cell instead of four 8-byte loads) when a
+    call samples at least CELL_GATHER_MIN_SAMPLES_PER_CELL times per cell and
+    the copy fits CELL_GATHER_MAX_BYTES; DPT_AMD_GATHER=linear|cells
+    overrides."""
+    if decoded.spec.precision != "float32":
         return False
+    override = os.environ.get("DPT_AMD_GATHER")
+    if override == "linear":
+        return False
+    cells = _native.cell_count(decoded.grid.shape)
     element = 4 if decoded.spec.storage == "float32" else 2
-    return 8 * element * _native.cell_count(decoded.grid.shape) <= CELL_GATHER_MAX_BYTES
+    if 8 * element * cells > CELL_GATHER_MAX_BYTES:
+        return False
+    if override == "cells":
+        return True
+    views, rows, cols = decoded.image_shape
+    return views * rows * cols * decoded.spec.samples_per_ray >= CELL_GATHER_MIN_SAMPLES_PER_CELL * cells
 
 
 def prepare(value: str, device: int, stream: int, needs: int) -> Prepared:

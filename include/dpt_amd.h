@@ -282,6 +282,15 @@ double dpt_kernel_ms(dpt_plan *plan, int32_t kind);
  * counterpart. */
 int64_t dpt_count_hit_rays(dpt_plan *plan, const double *poses, void *stream);
 
+/* VALU issue-rate probe: 16 independent chains of v_fma_f32 (kind 0) or
+ * v_add_u32 (kind 1) per lane, 8 waves per SIMD; *rate = wave64 VALU
+ * instructions per second over the device (one warm and one timed launch,
+ * HIP events on `stream`, synchronised).  scratch_out: device float buffer
+ * of 2048 * 256 elements.  The ceiling the projector kernels' instruction
+ * rates are reported against; measurement helper, no reference
+ * counterpart. */
+int dpt_valu_probe(int32_t kind, void *scratch_out, double *rate, void *stream);
+
 /* The launch layout a plan chose: out[0] FP64 LDS brick entries of the
  * volume adjoint, out[1] 1 when the FP64 volume adjoint chunks by slabs of
  * grid samples instead of runs of sample indices, out[2] samples per chunk,
