@@ -724,8 +724,9 @@ def cell_gauss_benchmark(grid, geometry, values, target, samples: int) -> dict:
         "rays_per_s_forward": round(rays / forward_ms * 1e3, 1),
         "midpoint_quadrature_rel_l2": quadrature,
         "midpoint_samples": samples,
-        "note": "FP64 exact rule; the volume VJP keeps the 4 corner sums a piece shares with the next piece's "
-                "cell in registers and adds the others with FP64 global atomics (no LDS privatisation)",
+        "note": "FP64 exact rule; the volume VJP walks lockstep chunks of the rays, adds each piece's 8 "
+                "FP32-rounded corner increments to the chunk's FP64 LDS brick and drains it with one FP64 "
+                "global atomic per touched grid sample and chunk",
     }
     _log(f"cell-gauss: forward {result['forward_ms']} ms, joint {result['forward_joint_vjp_ms']} ms, "
          f"midpoint quadrature error {quadrature:.2e}")

@@ -371,11 +371,12 @@ class Plan:
 
     def layout(self) -> dict:
         """The launch layout the plan chose (brick entries, slab chunks,
-        samples per chunk, views per VJP workgroup)."""
-        out = np.zeros(4, dtype=np.int32)
+        samples per chunk, views per VJP workgroup, cell-Gauss chunks per
+        ray)."""
+        out = np.zeros(5, dtype=np.int32)
         check(lib().dpt_plan_layout(self.handle, out.ctypes.data), "dpt_plan_layout")
         return {"brick_cap": int(out[0]), "slab_chunks": bool(out[1]), "chunk_samples": int(out[2]),
-                "views_per_block": int(out[3])}
+                "views_per_block": int(out[3]), "gauss_chunks": int(out[4])}
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:

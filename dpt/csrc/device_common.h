@@ -50,6 +50,7 @@ struct Params {
     int brick_cap;           // FP64 LDS brick entries per workgroup (volume adjoint)
     int slab_chunks;         // 1: the FP64 volume adjoint chunks by slabs of space, not by sample index
     int chunk_samples;       // samples per lockstep chunk of the volume adjoint
+    int gauss_chunks;        // lockstep chunks per ray of the cell-Gauss volume adjoint
     const float *alpha;      // 2 x samples entries: alpha_k, then k + 0.5 (sample_table_entries)
     int volume_f64;          // 1: FP64 volume destination (increments unflushed, FP64 sampling unrounded)
 };

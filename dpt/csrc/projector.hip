@@ -203,6 +203,16 @@ __device__ __forceinline__ void flush_device(const Params &P, const CellSums &cs
         if (cs.s[k] != 0.0f) unsafeAtomicAdd(&acc[base + corner_offset(P, k)], (double)cs.s[k]);
 }
 
+// Chunks per ray of the cell-Gauss volume adjoint: one per 55 voxels of the
+// grid diagonal, within [1, 64] -- 8 on a 256^3 grid, measured best against
+// 12 / 16 / 24 / 32 on a 360-view 512^2 full scan (joint VJP 418 / 423 / 429
+// / 452 / 476 ms): each chunk is halved, up to three times, only while its
+// brick overflows LDS, and fewer chunks mean fewer bounds and drains.
+inline int gauss_chunks(int nx, int ny, int nz) {
+    const double diagonal = std::sqrt((double)nx * nx + (double)ny * ny + (double)nz * nz);
+    return (int)std::lround(std::fmin(64.0, std::fmax(1.0, diagonal / 55.0)));
+}
+
 // Samples per lockstep chunk of the volume adjoint: 32 at 1024 samples over
 // a 256^3 grid (measured best against 16/24/48/64), scaled so that a chunk
 // spans about the same distance in voxels -- 32 * (S / 1024) * (443 / d),
@@ -1390,77 +1400,52 @@ __device__ __forceinline__ bool add_increment(const Params &P, double *acc, int 
     return isfinite(rounded);
 }
 
-// Register sums of one cell's 8 corners for the cell-Gauss volume adjoint:
-// consecutive pieces of a ray lie in face-adjacent cells sharing 4 corners,
-// which stay in registers while the other 4 are flushed with FP64 global
-// atomics (half the atomics of one per corner and piece).  Only for grids
-// whose axes all have >= 2 samples (no clamped duplicate corners).
-struct CellCarry {
-    int l[3];  // carried cell; l[0] < 0 when empty
-    double v[8];
+// Destination of the cell-Gauss volume increments during one chunk: the
+// workgroup's LDS brick when the chunk's cells fit it, else the FP64 device
+// accumulator.  A cell outside the chunk's bound (a crossing rounded to the
+// other side of a plane) also goes to the device accumulator.  Only for grids
+// whose axes all have >= 2 samples (corner k of a cell is its lower corner
+// plus the bits of k).
+struct GaussSink {
+    Brick B;
+    bool lds;
 
-    __device__ __forceinline__ void reset() {
-        l[0] = -1;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = 0.0;
-    }
-
-    __device__ __forceinline__ void flush(const Params &P, double *acc) {
-        if (l[0] >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (v[k] != 0.0) unsafeAtomicAdd(acc + corner_of(P, l, k), v[k]);
-        }
-        reset();
-    }
-
-    // the next cell is l + dir * e_A: corners on the far face survive
-    template <int A>
-    __device__ __forceinline__ void shift(const Params &P, double *acc, int dir) {
-        constexpr int bit = 1 << A;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (k & bit) continue;
-            const int keep = dir > 0 ? (k | bit) : k, drop = dir > 0 ? k : (k | bit);
-            if (v[drop] != 0.0) unsafeAtomicAdd(acc + corner_of(P, l, drop), v[drop]);
-            const double kept = v[keep];
-            v[k] = dir > 0 ? kept : 0.0;
-            v[k | bit] = dir > 0 ? 0.0 : kept;
-        }
-        l[A] += dir;
-    }
-
-    __device__ __forceinline__ void move_to(const Params &P, double *acc, const int *cell) {
-        if (l[0] < 0) {
-            l[0] = cell[0];
-            l[1] = cell[1];
-            l[2] = cell[2];
-            return;
-        }
-        const int dx = cell[0] - l[0], dy = cell[1] - l[1], dz = cell[2] - l[2];
-        if (dx == 0 && dy == 0 && dz == 0) return;
-        if (dy == 0 && dz == 0 && (dx == 1 || dx == -1)) {
-            shift<0>(P, acc, dx);
-        } else if (dx == 0 && dz == 0 && (dy == 1 || dy == -1)) {
-            shift<1>(P, acc, dy);
-        } else if (dx == 0 && dy == 0 && (dz == 1 || dz == -1)) {
-            shift<2>(P, acc, dz);
+    __device__ __forceinline__ void add(const Params &P, double *brick, double *acc, const int *l, int k,
+                                        double v) const {
+        const int x = l[0] - B.x0, y = l[1] - B.y0, z = l[2] - B.z0;
+        if (lds && (unsigned)x <= (unsigned)(B.bx - 2) && (unsigned)y <= (unsigned)(B.by - 2) &&
+            (unsigned)z <= (unsigned)(B.bz - 2)) {
+            atomicAdd(brick + ((z + ((k >> 2) & 1)) * B.by + y + ((k >> 1) & 1)) * B.bx + x + (k & 1), v);
         } else {
-            flush(P, acc);
-            l[0] = cell[0];
-            l[1] = cell[1];
-            l[2] = cell[2];
+            unsafeAtomicAdd(acc + corner_of(P, l, k), v);
         }
     }
 };
 
+// Each chunk of the cell-Gauss volume adjoint can be halved this many times
+// (8 = 2^3 units per chunk) while its brick overflows LDS.
+constexpr int kGaussSplit = 8;
+
 // kernels/projection.py:873-1237 with cell_gauss (FP32 volume destination:
-// each increment rounded to FP32, summed here in FP64)
+// each increment rounded to FP32, summed here in FP64).
+//
+// The volume adjoint walks a workgroup's rays in lockstep chunks of their
+// clip intervals, P.gauss_chunks per ray: a lane takes the pieces that start
+// in its ray's chunk, whose cells lie between the cells of the chunk's end
+// points (lower cell coordinates are monotone along a line).  The workgroup
+// bounds those cells, privatises the box in an FP64 LDS brick when it fits
+// (a chunk is halved, down to 1/8, while it overflows), adds each piece's 8
+// rounded corner increments to it with LDS atomics and adds the non-zero
+// entries to the FP64 accumulator once per touched grid sample and chunk.
+// (Carrying the 4 corners consecutive pieces share in registers halves the
+// LDS atomics but costs more in divergent shifts than it saves.)
 template <typename T, bool VOL, bool POSE, int C>
 __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__restrict__ field,
                                                            const double *__restrict__ seeds,
                                                            double *__restrict__ acc,
                                                            double *__restrict__ partials) {
+    extern __shared__ double brick[];
+    __shared__ int bound_red[kBlock / 64][6];
     int view, row, col, wslot;
     bool wave_valid;
     if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
@@ -1468,160 +1453,207 @@ __global__ __launch_bounds__(kBlock) void vjp_gauss_kernel(Params P, const T *__
 #pragma unroll
     for (int c = 0; c < kMaxComponents; ++c) gradient[c] = 0.0;
     const double *pose = P.poses + 12 * (size_t)view;
+
+    // ---- per-ray setup
+    bool active = false, bad = false;
+    World w{};
+    Ray r{};
+    Interval64 iv{};
+    double seed = 0.0, span = 0.0;
     if (ray_in_range(P, view, row, col)) {
         const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
-        World w;
         ray_world(P, pose, view, row, col, w);
-        Ray r;
         ray_grid(P, w, r);
         if (!r.valid) {
             flag(P, view, DPT_STATUS_RANGE);
         } else {
-            const Interval64 iv = interval64(P, r.o, r.d);
-            double span = iv.upper - iv.lower;
-            const double seed = seeds[ray];
-            if (seed == 0.0) span = 0.0;
-            if (span > 0.0) {
-                const double factor = seed * r.len;
-                double og[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
-                double integral = 0.0;  // sum of radius x node value (ray coordinates)
-                const bool carried = P.nx > 1 && P.ny > 1 && P.nz > 1;
-                CellCarry carry;
-                carry.reset();
-                CellWalk walk;
-                begin_cells(P, r, iv, walk);
-                double start = iv.lower;
-                const long long limit = (long long)P.nx + P.ny + P.nz + 1;
-                bool bad = false;
-                for (long long segments = 0; start < iv.upper && segments < limit; ++segments) {
-                    CellPiece s;
-                    cell_piece(P, r, walk, start, s);
-                    if (s.end <= start) {
-                        bad = true;
-                        break;
+            iv = interval64(P, r.o, r.d);
+            span = iv.upper - iv.lower;
+            seed = seeds[ray];
+            active = seed != 0.0 && span > 0.0;
+        }
+    }
+    const double factor = seed * r.len;
+    double og[3] = {0.0, 0.0, 0.0}, dg[3] = {0.0, 0.0, 0.0};
+    double integral = 0.0;  // sum of radius x node value (ray coordinates)
+    const bool carried = P.nx > 1 && P.ny > 1 && P.nz > 1;
+    CellWalk walk{};
+    double start = iv.lower;
+    bool walking = active;
+    if (active) begin_cells(P, r, iv, walk);
+    const long long limit = (long long)P.nx + P.ny + P.nz + 1;
+    long long segments = 0;
+
+    if constexpr (VOL) {
+        for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
+    }
+    // chunk bounds in units of 1 / units of each ray's clip interval
+    const int units = (VOL && carried) ? P.gauss_chunks * kGaussSplit : 1;
+    for (int u0 = 0, u1 = 0; u0 < units; u0 = u1) {
+        GaussSink sink{};
+        u1 = units;
+        if (VOL && carried) {
+            for (int len = kGaussSplit;; len >>= 1) {
+                u1 = min(u0 + len, units);
+                int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+                if (walking) {
+                    const double ta = iv.lower + span * ((double)u0 / (double)units);
+                    const double tb = u1 == units ? iv.upper : iv.lower + span * ((double)u1 / (double)units);
+                    const double qa[3] = {r.o[0] + ta * r.d[0], r.o[1] + ta * r.d[1], r.o[2] + ta * r.d[2]};
+                    const double qb[3] = {r.o[0] + tb * r.d[0], r.o[1] + tb * r.d[1], r.o[2] + tb * r.d[2]};
+                    int la[3], lb[3];
+                    lower_cell64(P, qa, la);
+                    lower_cell64(P, qb, lb);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = min(la[a], lb[a]);
+                        nhi[a] = -max(la[a], lb[a]);
                     }
-                    bad |= !finite3d(s.first) || !finite3d(s.second);
-                    if constexpr (POSE) {
-                        double g1[3], g2[3], v1, v2;
-                        if (s.shared) {  // one load of the cell for both nodes
-                            double v[8];
-                            load8d<T>(P, field, (s.lower[2] * P.ny + s.lower[1]) * P.nx + s.lower[0], v);
-                            v1 = node_in_cell<true>(P, s.first, s.lower, v, g1);
-                            v2 = node_in_cell<true>(P, s.second, s.lower, v, g2);
-                        } else {
-                            v1 = node_value<T, true>(P, field, s.first, nullptr, g1);
-                            v2 = node_value<T, true>(P, field, s.second, nullptr, g2);
-                        }
-                        if constexpr (C == kRayCoordinates) {
-                            integral += s.radius * v1;
-                            integral += s.radius * v2;
-                        }
+                }
+                bool overflow = false;
+                sink.lds = chunk_brick(P, bound_red, lo, nhi, false, sink.B, &overflow);
+                if (!overflow || len <= 1) break;
+                __syncthreads();  // the next bound rewrites bound_red
+            }
+        }
+        const double stop = u1 == units ? iv.upper : iv.lower + span * ((double)u1 / (double)units);
+        if (walking) {
+            for (; start < stop && segments < limit; ++segments) {
+                CellPiece s;
+                cell_piece(P, r, walk, start, s);
+                if (s.end <= start) {
+                    bad = true;
+                    walking = false;
+                    break;
+                }
+                bad |= !finite3d(s.first) || !finite3d(s.second);
+                if constexpr (POSE) {
+                    double g1[3], g2[3], v1, v2;
+                    if (s.shared) {  // one load of the cell for both nodes
+                        double v[8];
+                        load8d<T>(P, field, (s.lower[2] * P.ny + s.lower[1]) * P.nx + s.lower[0], v);
+                        v1 = node_in_cell<true>(P, s.first, s.lower, v, g1);
+                        v2 = node_in_cell<true>(P, s.second, s.lower, v, g2);
+                    } else {
+                        v1 = node_value<T, true>(P, field, s.first, nullptr, g1);
+                        v2 = node_value<T, true>(P, field, s.second, nullptr, g2);
+                    }
+                    if constexpr (C == kRayCoordinates) {
+                        integral += s.radius * v1;
+                        integral += s.radius * v2;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        og[a] += s.radius * g1[a];
+                        og[a] += s.radius * g2[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        dg[a] += s.radius * s.first_position * g1[a];
+                        dg[a] += s.radius * s.second_position * g2[a];
+                    }
+                }
+                if constexpr (VOL) {
+                    double vf = factor * s.radius;
+                    if (!isfinite(factor)) vf = product3(seed, r.len, s.radius);
+                    if (s.shared) {
+                        // kernels/projection.py:405-430 (scatter_cell)
+                        double f1[3], f2[3];
+                        const int n[3] = {P.nx, P.ny, P.nz};
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
-                            og[a] += s.radius * g1[a];
-                            og[a] += s.radius * g2[a];
+                            const double top = (double)(n[a] - 1);
+                            f1[a] = fmin(fmax(s.first[a], 0.0), top) - (double)s.lower[a];
+                            f2[a] = fmin(fmax(s.second[a], 0.0), top) - (double)s.lower[a];
                         }
+                        const double w1 = inside_support64(P, s.first) ? vf : 0.0;
+                        const double w2 = inside_support64(P, s.second) ? vf : 0.0;
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            dg[a] += s.radius * s.first_position * g1[a];
-                            dg[a] += s.radius * s.second_position * g2[a];
-                        }
-                    }
-                    if constexpr (VOL) {
-                        double vf = factor * s.radius;
-                        if (!isfinite(factor)) vf = product3(seed, r.len, s.radius);
-                        if (s.shared) {
-                            // kernels/projection.py:405-430 (scatter_cell)
-                            double f1[3], f2[3];
-                            const int n[3] = {P.nx, P.ny, P.nz};
+                        for (int k = 0; k < 8; ++k) {
+                            double first = w1, second = w2;
 #pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                const double top = (double)(n[a] - 1);
-                                f1[a] = fmin(fmax(s.first[a], 0.0), top) - (double)s.lower[a];
-                                f2[a] = fmin(fmax(s.second[a], 0.0), top) - (double)s.lower[a];
+                            for (int a = 0; a < 3; ++a) first *= ((k >> a) & 1) ? f1[a] : 1.0 - f1[a];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
+                            if (carried) {
+                                const double rounded = destination_increment(P, first + second);
+                                bad |= !isfinite(rounded);
+                                if (rounded != 0.0) sink.add(P, brick, acc, s.lower, k, rounded);
+                            } else {
+                                bad |= !add_increment(P, acc, corner_of(P, s.lower, k), first + second);
                             }
-                            const double w1 = inside_support64(P, s.first) ? vf : 0.0;
-                            const double w2 = inside_support64(P, s.second) ? vf : 0.0;
-                            if (carried) carry.move_to(P, acc, s.lower);
+                        }
+                    } else {
+                        // kernels/projection.py:437-470 (scatter per node)
+#pragma unroll
+                        for (int node = 0; node < 2; ++node) {
+                            const double *pt = node == 0 ? s.first : s.second;
+                            Cell64 c;
+                            if (!cell64(P, pt, c)) continue;
+                            const int lower[3] = {c.lx, c.ly, c.lz};
+                            const double f[3] = {c.fx, c.fy, c.fz};
 #pragma unroll
                             for (int k = 0; k < 8; ++k) {
-                                double first = w1, second = w2;
+                                double contribution = vf;
 #pragma unroll
-                                for (int a = 0; a < 3; ++a) first *= ((k >> a) & 1) ? f1[a] : 1.0 - f1[a];
-#pragma unroll
-                                for (int a = 0; a < 3; ++a) second *= ((k >> a) & 1) ? f2[a] : 1.0 - f2[a];
-                                if (carried) {
-                                    const double rounded = destination_increment(P, first + second);
-                                    bad |= !isfinite(rounded);
-                                    carry.v[k] += rounded;
-                                } else {
-                                    bad |= !add_increment(P, acc, corner_of(P, s.lower, k), first + second);
-                                }
-                            }
-                        } else {
-                            // kernels/projection.py:437-470 (scatter per node)
-#pragma unroll
-                            for (int node = 0; node < 2; ++node) {
-                                const double *pt = node == 0 ? s.first : s.second;
-                                Cell64 c;
-                                if (!cell64(P, pt, c)) continue;
-                                const int lower[3] = {c.lx, c.ly, c.lz};
-                                const double f[3] = {c.fx, c.fy, c.fz};
-#pragma unroll
-                                for (int k = 0; k < 8; ++k) {
-                                    double contribution = vf;
-#pragma unroll
-                                    for (int a = 0; a < 3; ++a) contribution *= ((k >> a) & 1) ? f[a] : 1.0 - f[a];
-                                    bad |= !add_increment(P, acc, corner_of(P, lower, k), contribution);
-                                }
+                                for (int a = 0; a < 3; ++a) contribution *= ((k >> a) & 1) ? f[a] : 1.0 - f[a];
+                                bad |= !add_increment(P, acc, corner_of(P, lower, k), contribution);
                             }
                         }
                     }
-                    start = s.end;
-                    advance_cells(P, r, walk, s.end, iv.upper);
                 }
-                if constexpr (VOL) carry.flush(P, acc);
-                bad |= start < iv.upper;
-                if (bad) flag(P, view, DPT_STATUS_RANGE);
-                if constexpr (POSE) {
-                    // kernels/projection.py:1031-1046: endpoint terms of the
-                    // zero-extended support at the clamped clip points
-                    double lo_pt[3], up_pt[3];
-                    const int n[3] = {P.nx, P.ny, P.nz};
+                start = s.end;
+                advance_cells(P, r, walk, s.end, iv.upper);
+            }
+        }
+        if constexpr (VOL) {
+            if (sink.lds) {
+                __syncthreads();
+                drain_brick(P, sink.B, brick, acc);
+            }
+            __syncthreads();
+        }
+    }
+    if (active) {
+        bad |= start < iv.upper;
+        if constexpr (POSE) {
+            // kernels/projection.py:1031-1046: endpoint terms of the
+            // zero-extended support at the clamped clip points
+            double lo_pt[3], up_pt[3];
+            const int n[3] = {P.nx, P.ny, P.nz};
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const double hi = (double)n[a] - 0.5;
-                        lo_pt[a] = fmin(fmax(r.o[a] + iv.lower * r.d[a], -0.5), hi);
-                        up_pt[a] = fmin(fmax(r.o[a] + iv.upper * r.d[a], -0.5), hi);
-                    }
-                    const double lower_value = node_value<T, false>(P, field, lo_pt, nullptr, nullptr);
-                    const double upper_value = node_value<T, false>(P, field, up_pt, nullptr, nullptr);
+            for (int a = 0; a < 3; ++a) {
+                const double hi = (double)n[a] - 0.5;
+                lo_pt[a] = fmin(fmax(r.o[a] + iv.lower * r.d[a], -0.5), hi);
+                up_pt[a] = fmin(fmax(r.o[a] + iv.upper * r.d[a], -0.5), hi);
+            }
+            const double lower_value = node_value<T, false>(P, field, lo_pt, nullptr, nullptr);
+            const double upper_value = node_value<T, false>(P, field, up_pt, nullptr, nullptr);
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0, lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
-                        const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0, up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
-                        og[a] += upper_value * up_o - lower_value * lo_o;
-                        dg[a] += upper_value * up_d - lower_value * lo_d;
-                    }
-                    if (isfinite(factor)) {
-                        pose_epilogue<C>(P, pose, w, factor, og, dg, gradient, seed, integral, true);
-                    } else {
-                        double sa[3], da[3];
+            for (int a = 0; a < 3; ++a) {
+                const double lo_o = (iv.lo_axis == a) ? iv.lo_o : 0.0, lo_d = (iv.lo_axis == a) ? iv.lo_d : 0.0;
+                const double up_o = (iv.up_axis == a) ? iv.up_o : 0.0, up_d = (iv.up_axis == a) ? iv.up_d : 0.0;
+                og[a] += upper_value * up_o - lower_value * lo_o;
+                dg[a] += upper_value * up_d - lower_value * lo_d;
+            }
+            if (isfinite(factor)) {
+                pose_epilogue<C>(P, pose, w, factor, og, dg, gradient, seed, integral, true);
+            } else {
+                double sa[3], da[3];
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            const double sg = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
-                            const double rg = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
-                            sa[i] = product4(seed, r.len, 1.0, sg);
-                            da[i] = product4(seed, r.len, 1.0, rg);
-                        }
-                        Params unit = P;
-                        for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
-                        pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient, seed, integral, true);
-                    }
+                for (int i = 0; i < 3; ++i) {
+                    const double sg = P.g[i] * og[0] + P.g[3 + i] * og[1] + P.g[6 + i] * og[2];
+                    const double rg = P.g[i] * dg[0] + P.g[3 + i] * dg[1] + P.g[6 + i] * dg[2];
+                    sa[i] = product4(seed, r.len, 1.0, sg);
+                    da[i] = product4(seed, r.len, 1.0, rg);
                 }
+                Params unit = P;
+                for (int i = 0; i < 9; ++i) unit.g[i] = (i % 4 == 0) ? 1.0 : 0.0;
+                pose_epilogue<C>(unit, pose, w, 1.0, sa, da, gradient, seed, integral, true);
             }
         }
     }
+    if (bad) flag(P, view, DPT_STATUS_RANGE);
     if constexpr (POSE) store_gradient<C>(P, gradient, partials, wslot, wave_valid, view, row, col);
 }
 
@@ -2026,12 +2058,13 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
             launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
     } else if (p->cell_gauss) {
         const double *sd = static_cast<const double *>(seeds);
+        const size_t lds = (size_t)P.brick_cap * sizeof(double);
         double *partials = pose_destination<C>(p);
         if (vol && pose)
-            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, true, C>), grid, block, lds, s, P, f, sd, p->d_acc,
                                partials);
         else if (vol)
-            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, false, C>), grid, block, 0, s, P, f, sd, p->d_acc,
+            hipLaunchKernelGGL((vjp_gauss_kernel<T, true, false, C>), grid, block, lds, s, P, f, sd, p->d_acc,
                                partials);
         else
             hipLaunchKernelGGL((vjp_gauss_kernel<T, false, true, C>), grid, block, 0, s, P, f, sd, p->d_acc,
@@ -2226,6 +2259,11 @@ int plan_create(const dpt_grid_desc *grid, int32_t views, int32_t rows, int32_t 
     if (const char *chunk = std::getenv("DPT_AMD_CHUNK")) {
         const int forced = std::atoi(chunk);
         if (forced >= 1 && forced <= 1024) P.chunk_samples = forced;
+    }
+    P.gauss_chunks = gauss_chunks(grid->nx, grid->ny, grid->nz);
+    if (const char *chunks = std::getenv("DPT_AMD_GAUSS_CHUNKS")) {
+        const int forced = std::atoi(chunks);
+        if (forced >= 1 && forced <= 256) P.gauss_chunks = forced;
     }
     P.slab_chunks = 0;
     P.brick_cap = P.bundle ? kBrickCapDefault : choose_brick_cap(P, panels, &P.slab_chunks);
@@ -2704,6 +2742,7 @@ int dpt_plan_layout(const dpt_plan *p, int32_t *out) {
     out[1] = p->base.slab_chunks;
     out[2] = p->base.chunk_samples;
     out[3] = p->vjp_vb;
+    out[4] = p->base.gauss_chunks;
     return DPT_OK;
 }
 

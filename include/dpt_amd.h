@@ -294,8 +294,9 @@ int dpt_valu_probe(int32_t kind, void *scratch_out, double *rate, void *stream);
 /* The launch layout a plan chose: out[0] FP64 LDS brick entries of the
  * volume adjoint, out[1] 1 when the FP64 volume adjoint chunks by slabs of
  * grid samples instead of runs of sample indices, out[2] samples per chunk,
- * out[3] views per workgroup of the VJP launches.  Inspection helper, no
- * reference counterpart. */
+ * out[3] views per workgroup of the VJP launches, out[4] chunks per ray of
+ * the cell-Gauss volume adjoint.  Inspection helper, no reference
+ * counterpart. */
 int dpt_plan_layout(const dpt_plan *plan, int32_t *out);
 
 const char *dpt_last_error(void);

@@ -75,6 +75,42 @@ def test_cell_gauss_vjp_matches_the_oracle(name, components):
     np.testing.assert_allclose(p_only, p_ref, rtol=1e-10, atol=1e-12 * max(1.0, np.abs(p_ref).max()))
 
 
+@pytest.mark.parametrize("mode", ["both", "volume"])
+@pytest.mark.parametrize("cap,chunks", [(None, None), ("512", "1"), ("512", "64"), (None, "1"), (None, "64"),
+                                        ("8192", "3")])
+def test_cell_gauss_volume_bricks_match_the_oracle(cap, chunks, mode, monkeypatch):
+    """The volume adjoint's lockstep chunks and LDS bricks: the plan's own
+    chunking, one chunk per ray, 64 chunks (shorter than a cell), the largest
+    brick, and the smallest (512 entries), where chunks overflow, are halved
+    and at 1/8 add to the device accumulator."""
+    print(f"case cell-gauss bricks cap={cap} chunks={chunks} {mode}", flush=True)
+    if cap is not None:
+        monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
+    if chunks is not None:
+        monkeypatch.setenv("DPT_AMD_GAUSS_CHUNKS", chunks)
+    n = 24
+    grid = O.Grid((n, n + 2, n + 4), (1.0, 0.9, 1.1), (-11.0, -12.0, -13.0))
+    panels = scenes.cone_panels(grid, 5, 20, 28, sod=120.0, sdd=200.0, offset=0.3)
+    values = scenes.smooth_field(grid, 6)
+    pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    ref = O.Scene(grid, 7, panels=panels)
+    gpu = NativeScene(grid, 7, panels=panels, precision="float64", extra_needs=_native.CELL_GAUSS)
+    layout = gpu.plan.layout()
+    assert layout["gauss_chunks"] == (int(chunks) if chunks is not None else 1)
+    assert layout["brick_cap"] == (int(cap) if cap is not None else layout["brick_cap"])
+    seeds = np.random.default_rng(12).uniform(-1.0, 1.0, int(np.prod(gpu.shape)))
+    seeds[::5] = 0.0
+    pose_grad = mode == "both"
+    g_ref, p_ref, _, ref_status = ref.vjp(values, seeds, pose, precision="fp64", pose=pose_grad,
+                                          integration="cell_gauss")
+    g, p, status = gpu.vjp(values, seeds, pose, pose_grad=pose_grad)
+    assert status.tolist() == ref_status.tolist() == [0] * gpu.shape[0]
+    assert np.count_nonzero(g_ref) > 0
+    np.testing.assert_allclose(g, g_ref, rtol=2e-7, atol=1e-12 * np.abs(g_ref).max())
+    if pose_grad:
+        np.testing.assert_allclose(p, p_ref, rtol=1e-10, atol=1e-12 * np.abs(p_ref).max())
+
+
 def test_projector_cell_gauss_is_exact_and_differentiable():
     """Projector(precision="fp64", integration="cell_gauss"): a constant field
     integrates to value x clipped length at any sample count; the midpoint

@@ -167,8 +167,8 @@ def test_fp32_vjp_matches_oracle(name, mode, components):
 @pytest.mark.parametrize("cap,chunk", [("512", None), ("512", "7"), (None, "1"), (None, "64"), ("8192", "16")])
 def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch):
     """LDS bricks of every size down to the smallest capacity (512 entries),
-    where most chunks overflow and take the device-atomic path with the
-    shared-face carry; chunk lengths from 1 to 64 samples."""
+    where most chunks overflow and take the device-atomic path; chunk
+    lengths from 1 to 64 samples."""
     print(f"case bricks cap={cap} chunk={chunk} {precision}", flush=True)
     if cap is not None:
         monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
