@@ -9,11 +9,12 @@ namespace dpt {
 // One ray of the gather, prepared once per call: the FP64 ray of the forward
 // march (kernels/projection.py:79-121) with its clipped interval, sample
 // step and volume factor (kernels/projection_adjoint.py:94-157), plus the
-// reciprocals the culling uses.  vf == 0 marks a ray that adds nothing
-// (invalid, zero cotangent or empty interval).
+// reciprocals and the box widening the culling uses.  vf == 0 marks a ray
+// that adds nothing (invalid, zero cotangent or empty interval).
 struct alignas(16) GatherRay {
     double o[3], d[3], inv[3];  // grid-frame origin, direction, 1/direction (0 where direction is 0)
-    double lower, step, inv_step, vf, pad;
+    double lower, step, inv_step, vf;
+    double margin;              // support-box widening for this ray's sample points, grid units
 };
 static_assert(sizeof(GatherRay) == 112, "a ray record is seven 16-byte loads");
 

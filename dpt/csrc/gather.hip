@@ -27,6 +27,7 @@
 // pixel.
 #include "gather.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace dpt {
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void gather_rays_kernel(Params P, const dou
         ray_grid(P, w, r);
         GatherRay g;
         for (int a = 0; a < 3; ++a) g.o[a] = g.d[a] = g.inv[a] = 0.0;
-        g.lower = g.step = g.inv_step = g.vf = g.pad = 0.0;
+        g.lower = g.step = g.inv_step = g.vf = g.margin = 0.0;
         if (!r.valid) {
             flag(P, view, DPT_STATUS_RANGE);
         } else {
@@ -259,12 +260,16 @@ __global__ __launch_bounds__(kBlock) void gather_rays_kernel(Params P, const dou
                 for (int a = 0; a < 3; ++a) {
                     g.o[a] = r.o[a];
                     g.d[a] = r.d[a];
-                    g.inv[a] = r.d[a] != 0.0 ? 1.0 / r.d[a] : 0.0;
+                    g.inv[a] = 1.0 / r.d[a];  // +-inf for a zero component
                 }
                 g.lower = iv.lower;
                 g.step = step;
                 g.inv_step = 1.0 / step;
                 g.vf = vf;
+                // the rounding of a sample point is below 1e-15 of |o| + |d|
+                double reach = 0.0;
+                for (int a = 0; a < 3; ++a) reach = fmax(reach, fabs(r.o[a]) + fabs(r.d[a]));
+                g.margin = kBoxMargin * (1.0 + reach);
             }
         }
         rays[i] = g;
@@ -273,6 +278,7 @@ __global__ __launch_bounds__(kBlock) void gather_rays_kernel(Params P, const dou
 
 struct Voxel {
     int i[3];
+    double c[3];          // i as doubles
     double lo[3], hi[3];  // clamped-cell support: lower cell coordinate in [i-1, i] per axis
 };
 
@@ -357,39 +363,41 @@ __device__ __forceinline__ bool pixel_window(const Params &P, const GatherView &
 }
 
 // Sample indices [k0, k1] of ray g whose points can lie in the voxel's
-// support box; false when none can.
+// support box; false when none can.  Branch-free slab clip of the widened
+// box (lanes of a wave test different rays, so every lane runs the same
+// instructions).  A zero direction component has the reciprocal +-inf, so
+// its slab is by itself unbounded (the constant coordinate inside: t from
+// -inf to +inf) or empty (outside: both t at the same infinity); a widened
+// bound exactly at the coordinate gives 0 * inf = NaN, which fmin / fmax
+// ignore, leaving the range empty -- right, as that coordinate lies the
+// margin outside the unwidened box.
 __device__ __forceinline__ bool sample_window(const Params &P, const GatherRay &g, const Voxel &vx, int &k0,
                                               int &k1) {
-    // Branch-free slab clip of the widened box: lanes of a wave test
-    // different rays, so every lane runs the same instructions.
     double tlo = -INFINITY, thi = INFINITY;
-    bool outside = false, full = false;
+    const double m = g.margin;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const double m = kBoxMargin * (1.0 + fabs(g.o[a]) + fabs(g.d[a]));
-        const double lo = (vx.lo[a] - m) - g.o[a], hi = (vx.hi[a] + m) - g.o[a];
-        const bool flat = g.d[a] == 0.0;  // the coordinate is o on the whole ray
-        outside = outside || (flat && (lo > 0.0 || hi < 0.0));
-        const double t1 = lo * g.inv[a], t2 = hi * g.inv[a];
-        full = full || (!flat && (isnan(t1) || isnan(t2)));
-        tlo = flat ? tlo : fmax(tlo, fmin(t1, t2));
-        thi = flat ? thi : fmin(thi, fmax(t1, t2));
+        const double t1 = ((vx.lo[a] - m) - g.o[a]) * g.inv[a], t2 = ((vx.hi[a] + m) - g.o[a]) * g.inv[a];
+        tlo = fmax(tlo, fmin(t1, t2));
+        thi = fmin(thi, fmax(t1, t2));
     }
     tlo -= kParamMargin * (1.0 + fabs(tlo));
     thi += kParamMargin * (1.0 + fabs(thi));
+    // an empty range (infinite ends give NaN here and also count as empty)
+    const bool meets = tlo <= thi;
     // sample k lies in [tlo, thi] when kf0 <= k <= kf1; the box and t
     // margins already exceed the point rounding by orders of magnitude, the
     // slack only absorbs the rounding of kf itself
     const double slack = 1e-6 + 1e-14 * g.inv_step;
     double kf0 = (tlo - g.lower) * g.inv_step - 0.5 - slack, kf1 = (thi - g.lower) * g.inv_step - 0.5 + slack;
-    const bool exact = !full && g.step > kMinStep && !isnan(kf0) && !isnan(kf1);
+    const bool exact = g.step > kMinStep && !isnan(kf0) && !isnan(kf1);
     const double top = (double)P.samples + 1.0;
     kf0 = fmin(fmax(kf0, -2.0), top);
     kf1 = fmin(fmax(kf1, -2.0), top);
     const int a0 = max(0, (int)ceil(kf0)), a1 = min(P.samples - 1, (int)floor(kf1));
     k0 = exact ? a0 : 0;
     k1 = exact ? a1 : P.samples - 1;
-    return !outside && (!exact || (tlo <= thi && a0 <= a1));
+    return meets && (!exact || a0 <= a1);
 }
 
 // One ray record in seven 16-byte loads, issued together.
@@ -410,7 +418,7 @@ __device__ __forceinline__ GatherRay load_ray(const GatherRay *r) {
     g.step = q5.x;
     g.inv_step = q5.y;
     g.vf = q6.x;
-    g.pad = q6.y;
+    g.margin = q6.y;
     return g;
 }
 
@@ -420,40 +428,62 @@ __device__ __forceinline__ GatherRay load_ray(const GatherRay *r) {
 //             each compared with the voxel (clamped duplicates on singleton
 //             axes are added once per corner, as in the reference);
 //   kSimple   every axis has >= 2 samples: at most one corner is the voxel;
-//   kInterior every coordinate of the voxel is in [2, n-3]: a matching
-//             cell lies in [1, n-2) on every axis, where the support test
-//             and the clamps are the identity, and a point outside the
-//             support or clamped gives a lower corner of -1 or less, 0 or
-//             n-2 on some axis, none of which can match; so floor(p)
-//             replaces the clamped cell and the support test is dropped.
-constexpr int kGeneral = 0, kSimple = 1, kInterior = 2;
+//   kMasked   every axis has >= 2 samples and the ray's factor is finite:
+//             the product of three per-axis weights is added for every
+//             sample, a weight being 0 where the reference finds no match.
+//             On an axis of MASK (the voxel coordinate i in [2, n-3] for
+//             every lane of the wave) a matching cell lies in [1, n-2),
+//             where the support test and the clamps are the identity, and a
+//             point outside the support or clamped gives a lower corner of
+//             -1 or less, 0 or n-2, none of which can match.  With
+//             e = p - i the lower corner floor(p) is i - 1 or i exactly when
+//             -1 <= e < 1, and the reference's weight (1 - f for lower
+//             corner i, f = p - (i - 1) for i - 1) is 1 - |e| bit for bit:
+//             p - i and p - (i - 1) are exact there (Sterbenz, i >= 2), so
+//             1 - |e| is the representable f or the same 1 - f; every other
+//             point has max(1 - |e|, 0) = 0 (|e| >= 1, or a point just
+//             below i - 1 whose e rounds to -1).  Other axes take the
+//             support test, clamps and match of kSimple.  A sample without
+//             a match adds a signed zero, which leaves the sum unchanged (a
+//             running sum that starts at +0 is never -0).
+constexpr int kGeneral = 0, kSimple = 1, kMasked = 2;
 
 // Add the contributions of ray g's samples k0..k1 to the voxel's total, in
 // sample then corner order (kernels/projection_adjoint.py:158-163 with
 // point_contribution, :33-44).
-template <int MODE>
+template <int MODE, int MASK = 0>
 __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, const Voxel &vx, int k0, int k1,
                                           double total) {
     const int n[3] = {P.nx, P.ny, P.nz};
     const int cap[3] = {P.capx, P.capy, P.capz};
     const double vf = g.vf;
+    if constexpr (MODE == kMasked) {
+        double kc = (double)k0 + 0.5;  // k + 0.5, exact
+        for (int k = k0; k <= k1; ++k, kc += 1.0) {
+            const double pos = g.lower + kc * g.step;
+            double w[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double p = g.o[a] + pos * g.d[a];
+                if ((MASK >> a) & 1) {
+                    w[a] = fmax(1.0 - fabs(p - vx.c[a]), 0.0);
+                } else {
+                    // kernels/projection.py:124-134, 199-214
+                    const bool inside = p >= -0.5 && p <= (double)n[a] - 0.5;
+                    const double c = fmin(fmax(p, 0.0), (double)(n[a] - 1));
+                    const int l = min((int)floor(c), cap[a]);
+                    const double f = c - (double)l;
+                    w[a] = !inside ? 0.0 : l == vx.i[a] ? 1.0 - f : l + 1 == vx.i[a] ? f : 0.0;
+                }
+            }
+            total += ((vf * w[0]) * w[1]) * w[2];
+        }
+        return total;
+    }
     for (int k = k0; k <= k1; ++k) {
         const double pos = g.lower + ((double)k + 0.5) * g.step;
         double p[3];
         for (int a = 0; a < 3; ++a) p[a] = g.o[a] + pos * g.d[a];
-        if constexpr (MODE == kInterior) {
-            bool hit = true;
-            double w[3];
-            for (int a = 0; a < 3; ++a) {
-                const double lower = floor(p[a]);
-                const int l = (int)lower;
-                const double f = p[a] - lower;
-                hit = hit && (l == vx.i[a] || l + 1 == vx.i[a]);
-                w[a] = l == vx.i[a] ? 1.0 - f : f;
-            }
-            if (hit) total += ((vf * w[0]) * w[1]) * w[2];
-            continue;
-        }
         // kernels/projection.py:124-134 (inside_support; NaN fails every test)
         if (!(p[0] >= -0.5 && p[1] >= -0.5 && p[2] >= -0.5 && p[0] <= (double)n[0] - 0.5 &&
               p[1] <= (double)n[1] - 0.5 && p[2] <= (double)n[2] - 0.5))
@@ -492,7 +522,7 @@ __device__ __forceinline__ double add_ray(const Params &P, const GatherRay &g, c
 }
 
 // The voxel's FP64 sum over every view, pixel window and sample window.
-template <int MODE>
+template <int MODE, int MASK = 0>
 __device__ __forceinline__ double gather_voxel(const Params &P, const GatherRay *__restrict__ rays,
                                                const GatherView *__restrict__ views, const Voxel &vx) {
     const int64_t pixels = (int64_t)P.rows * P.cols;
@@ -506,34 +536,119 @@ __device__ __forceinline__ double gather_voxel(const Params &P, const GatherRay 
             r0 = 0;
             r1 = gv.rows - 1;
         }
-        const GatherRay *base = rays + (int64_t)view * pixels;
-        for (int row = r0; row <= r1; ++row)
-            for (int col = c0; col <= c1; ++col) {
-                const GatherRay g = load_ray(base + (int64_t)row * gv.cols + col);
-                int k0, k1;
-                const bool reach = sample_window(P, g, vx, k0, k1);
-                if (g.vf != 0.0 && reach) total = add_ray<MODE>(P, g, vx, k0, k1, total);
+        // the window's pixels row-major in one loop, each ray record loaded
+        // one pixel ahead so that its latency overlaps the current ray
+        const int width = c1 - c0 + 1;
+        const int count = (c1 >= c0 && r1 >= r0) ? width * (r1 - r0 + 1) : 0;
+        const GatherRay *line = rays + (int64_t)view * pixels + (int64_t)r0 * gv.cols;
+        int col = c0 - 1;
+        for (int j = 0; j < count; ++j) {
+            if (++col > c1) {
+                col = c0;
+                line += gv.cols;
             }
+            const GatherRay g = load_ray(line + col);
+            int k0, k1;
+            const bool reach = sample_window(P, g, vx, k0, k1);
+            if (g.vf != 0.0 && reach) {
+                if (MODE == kMasked && !isfinite(g.vf))
+                    total = add_ray<kSimple>(P, g, vx, k0, k1, total);
+                else
+                    total = add_ray<MODE, MASK>(P, g, vx, k0, k1, total);
+            }
+        }
     }
     return total;
 }
 
+// Brick order of the launch.  Voxels within two samples of a grid face get
+// every sample of the short rays that graze that face or its edges -- tens
+// of times the samples of an interior voxel -- so the bricks on the surface
+// of the brick grid go first, one to each XCD in turn, and run beside the
+// rest instead of after it.  The interior bricks follow, each XCD taking a
+// contiguous range so that its L2 keeps the ray records of neighbouring
+// bricks (workgroup b runs on XCD b % 8; surface is padded to a multiple of
+// 8 so that interior workgroups keep that correspondence).
+struct BrickLayout {
+    int nx, ny, nz;        // bricks per axis
+    int surface;           // bricks on the surface of the brick grid
+    int surface_padded;    // rounded up to a multiple of kXcds
+    int interior;          // (nx - 2)(ny - 2)(nz - 2), or 0
+    int chunk;             // interior bricks per XCD
+};
+
+BrickLayout brick_layout(const Params &P) {
+    BrickLayout L;
+    L.nx = (P.nx + 7) / 8;
+    L.ny = (P.ny + 7) / 8;
+    L.nz = (P.nz + 3) / 4;
+    const int layer = L.nx * L.ny;
+    const int ring = layer - std::max(L.nx - 2, 0) * std::max(L.ny - 2, 0);
+    L.surface = (L.nz >= 2 ? 2 * layer : layer) + std::max(L.nz - 2, 0) * ring;
+    L.surface_padded = (L.surface + kXcds - 1) / kXcds * kXcds;
+    L.interior = std::max(L.nx - 2, 0) * std::max(L.ny - 2, 0) * std::max(L.nz - 2, 0);
+    L.chunk = (L.interior + kXcds - 1) / kXcds;
+    return L;
+}
+
+// Brick of workgroup b; false for padding workgroups.
+__device__ __forceinline__ bool brick_of(const BrickLayout &L, int b, int &bx, int &by, int &bz) {
+    if (b < L.surface_padded) {
+        if (b >= L.surface) return false;
+        const int layer = L.nx * L.ny;
+        int q;
+        if (b < layer) {  // bottom layer
+            bz = 0;
+            q = b;
+        } else if (b >= L.surface - layer) {  // top layer (nz >= 2)
+            bz = L.nz - 1;
+            q = b - (L.surface - layer);
+        } else {  // the ring of a middle layer
+            const int ring = layer - max(L.nx - 2, 0) * max(L.ny - 2, 0);
+            const int r = b - layer;
+            bz = 1 + r / ring;
+            q = r - (bz - 1) * ring;
+            if (L.nx > 2 && L.ny > 2) {
+                // first row, last row, then the two side columns
+                if (q < L.nx) {
+                    bx = q;
+                    by = 0;
+                } else if (q < 2 * L.nx) {
+                    bx = q - L.nx;
+                    by = L.ny - 1;
+                } else {
+                    const int s = q - 2 * L.nx;
+                    by = 1 + (s >> 1);
+                    bx = (s & 1) ? L.nx - 1 : 0;
+                }
+                return true;
+            }
+        }
+        by = q / L.nx;
+        bx = q - by * L.nx;
+        return true;
+    }
+    const int i = b - L.surface_padded;
+    const int logical = (i % kXcds) * L.chunk + i / kXcds;
+    if (logical >= L.interior) return false;
+    const int ix = L.nx - 2, iy = L.ny - 2;
+    bx = 1 + logical % ix;
+    const int rest = logical / ix;
+    by = 1 + rest % iy;
+    bz = 1 + rest / iy;
+    return true;
+}
+
 // Voxel bricks of 8 x 8 x 4 per workgroup (4 x 4 x 4 per wave, so a wave's
-// voxels see nearly the same rays); consecutive workgroups go to different
-// XCDs, so each XCD takes a contiguous range of bricks and its L2 keeps the
-// ray records of neighbouring bricks.
+// voxels see nearly the same rays).
 template <typename OUT, bool SIMPLE>
 __global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRay *__restrict__ rays,
                                                         const GatherView *__restrict__ views,
                                                         const int *__restrict__ guard, OUT *__restrict__ out,
-                                                        int accumulate, int bricks_x, int bricks_y, int bricks,
-                                                        int chunk) {
+                                                        int accumulate, BrickLayout L) {
     if (*guard) return;  // a failed check: the destination stays untouched (uniform)
-    const int b = blockIdx.x;
-    const int logical = (b % kXcds) * chunk + b / kXcds;
-    if (logical >= bricks) return;  // uniform per workgroup
-    const int bx = logical % bricks_x, rest = logical / bricks_x;
-    const int by = rest % bricks_y, bz = rest / bricks_y;
+    int bx, by, bz;
+    if (!brick_of(L, blockIdx.x, bx, by, bz)) return;  // uniform per workgroup
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     Voxel vx;
     vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
@@ -542,18 +657,26 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRa
     const int n[3] = {P.nx, P.ny, P.nz};
     if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;  // no barriers below
     for (int a = 0; a < 3; ++a) {
+        vx.c[a] = (double)vx.i[a];
         vx.lo[a] = vx.i[a] <= 1 ? -0.5 : (double)(vx.i[a] - 1);
         vx.hi[a] = vx.i[a] >= n[a] - 2 ? (double)n[a] - 0.5 : (double)(vx.i[a] + 1);
     }
     double total;
     if constexpr (SIMPLE) {
-        bool interior = true;
-        for (int a = 0; a < 3; ++a) interior = interior && vx.i[a] >= 2 && vx.i[a] <= n[a] - 3;
-        // one path per wave: bricks on the grid boundary take the checked one
-        if (__all(interior))
-            total = gather_voxel<kInterior>(P, rays, views, vx);
-        else
-            total = gather_voxel<kSimple>(P, rays, views, vx);
+        // axes on which every voxel of the wave is at least two samples
+        // from both grid faces (uniform over the wave)
+        int mask = 0;
+        for (int a = 0; a < 3; ++a) mask |= __all(vx.i[a] >= 2 && vx.i[a] <= n[a] - 3) ? 1 << a : 0;
+        switch (mask) {
+            case 0: total = gather_voxel<kMasked, 0>(P, rays, views, vx); break;
+            case 1: total = gather_voxel<kMasked, 1>(P, rays, views, vx); break;
+            case 2: total = gather_voxel<kMasked, 2>(P, rays, views, vx); break;
+            case 3: total = gather_voxel<kMasked, 3>(P, rays, views, vx); break;
+            case 4: total = gather_voxel<kMasked, 4>(P, rays, views, vx); break;
+            case 5: total = gather_voxel<kMasked, 5>(P, rays, views, vx); break;
+            case 6: total = gather_voxel<kMasked, 6>(P, rays, views, vx); break;
+            default: total = gather_voxel<kMasked, 7>(P, rays, views, vx); break;
+        }
     } else {
         total = gather_voxel<kGeneral>(P, rays, views, vx);
     }
@@ -572,16 +695,12 @@ unsigned stride_blocks(int64_t n) {
 template <typename OUT>
 void launch_voxels(const Params &P, const GatherRay *rays, const GatherView *views, const int *guard, OUT *out,
                    int accumulate, hipStream_t s) {
-    const int bricks_x = (P.nx + 7) / 8, bricks_y = (P.ny + 7) / 8, bricks_z = (P.nz + 3) / 4;
-    const int bricks = bricks_x * bricks_y * bricks_z;
-    const int chunk = (bricks + kXcds - 1) / kXcds;
-    const dim3 grid((unsigned)(chunk * kXcds)), block(kBlock);
+    const BrickLayout L = brick_layout(P);
+    const dim3 grid((unsigned)(L.surface_padded + L.chunk * kXcds)), block(kBlock);
     if (P.nx > 1 && P.ny > 1 && P.nz > 1)
-        hipLaunchKernelGGL((gather_kernel<OUT, true>), grid, block, 0, s, P, rays, views, guard, out, accumulate,
-                           bricks_x, bricks_y, bricks, chunk);
+        hipLaunchKernelGGL((gather_kernel<OUT, true>), grid, block, 0, s, P, rays, views, guard, out, accumulate, L);
     else
-        hipLaunchKernelGGL((gather_kernel<OUT, false>), grid, block, 0, s, P, rays, views, guard, out, accumulate,
-                           bricks_x, bricks_y, bricks, chunk);
+        hipLaunchKernelGGL((gather_kernel<OUT, false>), grid, block, 0, s, P, rays, views, guard, out, accumulate, L);
 }
 
 }  // namespace
