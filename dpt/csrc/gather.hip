@@ -569,7 +569,18 @@ __device__ __forceinline__ double gather_voxel(const Params &P, const GatherRay 
 // contiguous range so that its L2 keeps the ray records of neighbouring
 // bricks (workgroup b runs on XCD b % 8; surface is padded to a multiple of
 // 8 so that interior workgroups keep that correspondence).
+//
+// The voxels along the grid's 12 edges (two coordinates within two samples
+// of a face) collect the most: every sample of every ray that clips that
+// edge, up to a hundred times an interior voxel.  They go before everything
+// else, each wave taking 64 consecutive voxels along one edge line: those
+// voxels' pixel windows advance through the same columns (or rows) in step,
+// so their long windows coincide in the wave instead of queueing one after
+// another.  The lanes of surface bricks that fall on an edge line are idle.
 struct BrickLayout {
+    int edges;             // edge waves (0: no edge section, a grid axis below 4 samples)
+    int zw, xw, yw;        // waves per edge line along z, x, y
+    int edge_groups;       // workgroups of the edge section, a multiple of kXcds
     int nx, ny, nz;        // bricks per axis
     int surface;           // bricks on the surface of the brick grid
     int surface_padded;    // rounded up to a multiple of kXcds
@@ -579,6 +590,17 @@ struct BrickLayout {
 
 BrickLayout brick_layout(const Params &P) {
     BrickLayout L;
+    L.edges = L.zw = L.xw = L.yw = L.edge_groups = 0;
+    if (P.nx >= 4 && P.ny >= 4 && P.nz >= 4) {
+        // 16 lines per edge family (4 x 4 band coordinates); lines along x
+        // and y leave out their band ends, which lie on the lines along z
+        // or at a corner
+        L.zw = (P.nz + 63) / 64;
+        L.xw = (P.nx - 4 + 63) / 64;
+        L.yw = (P.ny - 4 + 63) / 64;
+        L.edges = 16 * (L.zw + L.xw + L.yw);
+        L.edge_groups = ((L.edges + 3) / 4 + kXcds - 1) / kXcds * kXcds;
+    }
     L.nx = (P.nx + 7) / 8;
     L.ny = (P.ny + 7) / 8;
     L.nz = (P.nz + 3) / 4;
@@ -591,7 +613,47 @@ BrickLayout brick_layout(const Params &P) {
     return L;
 }
 
-// Brick of workgroup b; false for padding workgroups.
+// Band coordinate q in 0..3 of an axis of n >= 4 samples: 0, 1, n-2, n-1.
+__device__ __forceinline__ int band(int q, int n) { return q < 2 ? q : n - 4 + q; }
+
+__device__ __forceinline__ bool in_band(int i, int n) { return i <= 1 || i >= n - 2; }
+
+// A voxel on an edge line: two of its coordinates in the band.
+__device__ __forceinline__ bool on_edge(const int *i, const int *n) {
+    const bool x = in_band(i[0], n[0]), y = in_band(i[1], n[1]), z = in_band(i[2], n[2]);
+    return (x && y) || (y && z) || (x && z);
+}
+
+// Voxel of lane `lane` of edge wave e; false past the line's end.  Lines
+// along z take every z at band (x, y); lines along x take x in [2, n-3] at
+// band (y, z); lines along y take y in [2, n-3] at band (x, z).
+__device__ __forceinline__ bool edge_voxel(const BrickLayout &L, const int *n, int e, int lane, int *i) {
+    if (e >= L.edges) return false;
+    if (e < 16 * L.zw) {
+        const int line = e / L.zw, seg = e - line * L.zw;
+        i[0] = band(line & 3, n[0]);
+        i[1] = band(line >> 2, n[1]);
+        i[2] = seg * 64 + lane;
+        return i[2] < n[2];
+    }
+    e -= 16 * L.zw;
+    if (e < 16 * L.xw) {
+        const int line = e / L.xw, seg = e - line * L.xw;
+        i[1] = band(line & 3, n[1]);
+        i[2] = band(line >> 2, n[2]);
+        i[0] = 2 + seg * 64 + lane;
+        return i[0] < n[0] - 2;
+    }
+    e -= 16 * L.xw;
+    const int line = e / L.yw, seg = e - line * L.yw;
+    i[0] = band(line & 3, n[0]);
+    i[2] = band(line >> 2, n[2]);
+    i[1] = 2 + seg * 64 + lane;
+    return i[1] < n[1] - 2;
+}
+
+// Brick of workgroup b (counted after the edge section); false for padding
+// workgroups.
 __device__ __forceinline__ bool brick_of(const BrickLayout &L, int b, int &bx, int &by, int &bz) {
     if (b < L.surface_padded) {
         if (b >= L.surface) return false;
@@ -647,15 +709,21 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(Params P, const GatherRa
                                                         const int *__restrict__ guard, OUT *__restrict__ out,
                                                         int accumulate, BrickLayout L) {
     if (*guard) return;  // a failed check: the destination stays untouched (uniform)
-    int bx, by, bz;
-    if (!brick_of(L, blockIdx.x, bx, by, bz)) return;  // uniform per workgroup
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    Voxel vx;
-    vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
-    vx.i[1] = by * 8 + (w >> 1) * 4 + ((l >> 2) & 3);
-    vx.i[2] = bz * 4 + (l >> 4);
     const int n[3] = {P.nx, P.ny, P.nz};
-    if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;  // no barriers below
+    Voxel vx;
+    // no barriers below: lanes without a voxel return
+    if ((int)blockIdx.x < L.edge_groups) {
+        if (!edge_voxel(L, n, 4 * (int)blockIdx.x + w, l, vx.i)) return;
+    } else {
+        int bx, by, bz;
+        if (!brick_of(L, (int)blockIdx.x - L.edge_groups, bx, by, bz)) return;  // uniform per workgroup
+        vx.i[0] = bx * 8 + (w & 1) * 4 + (l & 3);
+        vx.i[1] = by * 8 + (w >> 1) * 4 + ((l >> 2) & 3);
+        vx.i[2] = bz * 4 + (l >> 4);
+        if (vx.i[0] >= n[0] || vx.i[1] >= n[1] || vx.i[2] >= n[2]) return;
+        if (L.edges > 0 && on_edge(vx.i, n)) return;  // gathered by the edge section
+    }
     for (int a = 0; a < 3; ++a) {
         vx.c[a] = (double)vx.i[a];
         vx.lo[a] = vx.i[a] <= 1 ? -0.5 : (double)(vx.i[a] - 1);
@@ -696,7 +764,7 @@ template <typename OUT>
 void launch_voxels(const Params &P, const GatherRay *rays, const GatherView *views, const int *guard, OUT *out,
                    int accumulate, hipStream_t s) {
     const BrickLayout L = brick_layout(P);
-    const dim3 grid((unsigned)(L.surface_padded + L.chunk * kXcds)), block(kBlock);
+    const dim3 grid((unsigned)(L.edge_groups + L.surface_padded + L.chunk * kXcds)), block(kBlock);
     if (P.nx > 1 && P.ny > 1 && P.nz > 1)
         hipLaunchKernelGGL((gather_kernel<OUT, true>), grid, block, 0, s, P, rays, views, guard, out, accumulate, L);
     else
