@@ -57,10 +57,13 @@ def _args():
     parser.add_argument("--serving-requests", type=int, default=200)
     parser.add_argument("--serving-rates", default="10,50",
                         help="Poisson arrival rates (requests/s) of the open-loop runs, comma separated")
+    parser.add_argument("--serving-forms", default="batched_bf16,batched_fp32,plain_fp32",
+                        help="serving forms for --serving-only (batched_bf16, batched_fp32, plain_fp32)")
     parser.add_argument("--serving-only", action="store_true",
                         help="only the serving measurement (every form at every rate; one JSON object)")
     parser.add_argument("--no-guards", action="store_true", help="skip the full-scan acceptance guards")
     parser.add_argument("--no-request", action="store_true", help="skip the single-request (chest AP) timings")
+    parser.add_argument("--request-only", action="store_true", help="only the single-request (chest AP) timings")
     parser.add_argument("--record-counters", nargs="+", default=None, metavar="CSV",
                         help="instead of the benchmark, write profiles/projector_counters.json from rocprofv3 "
                              "counter CSVs of --kernels-once runs (copy-calibration CSVs first, see "
@@ -232,8 +235,11 @@ def serving_workload(count: int, seed: int = 404):
         grid = phantoms.centred_grid(shape, spacing)
         volumes.append((f"vol{i:02d}", grid, phantoms.chest_phantom(grid, seed=100 + i)))
     requests, truths = [], []
+    on_device = {}
     for j in range(count):
         key, grid, values = volumes[int(rng.integers(16))]
+        if key not in on_device:
+            on_device[key] = torch.tensor(values, device="cuda:0")
         views = int(rng.integers(2, 9))
         det = int(rng.choice([128, 256, 512]))
         samples = int(rng.choice([256, 512, 768, 1024]))
@@ -249,10 +255,10 @@ def serving_workload(count: int, seed: int = 404):
         truth = compose_pose(RigidTransform(), tuple(xi)).packed()
         with torch.no_grad():
             image = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)(
-                torch.tensor(values, device="cuda:0"), packed_pose=torch.tensor(truth, dtype=torch.float64, device="cuda:0")
+                on_device[key], packed_pose=torch.tensor(truth, dtype=torch.float64, device="cuda:0")
             )
             image += 1e-3 * torch.randn(image.shape, generator=torch.Generator("cuda:0").manual_seed(j), device="cuda:0")
-        requests.append(serving.FitRequest(f"req{j:03d}", key, geometry, image.cpu().numpy(), samples, iterations))
+        requests.append(serving.FitRequest(f"req{j:04d}", key, geometry, image.cpu().numpy(), samples, iterations))
         truths.append(truth)
     return volumes, requests, truths
 
@@ -272,7 +278,8 @@ def serving_benchmark(count: int, rates, forms=("batched_bf16", "batched_fp32", 
     """Pose-fit service: each form closed-loop (all requests queued at t = 0)
     and, for ``open_forms``, open-loop at each Poisson arrival rate in
     ``rates``; ``count`` requests per form and per rate.  Latency is arrival
-    to completion; p50 / p99 over all requests of a run."""
+    to completion; p50 / p99 over all requests of a run.  A progress line
+    goes to stderr every 100 completions."""
     import dataclasses
 
     import numpy as np
@@ -288,7 +295,14 @@ def serving_benchmark(count: int, rates, forms=("batched_bf16", "batched_fp32", 
             registry.add(key, values, grid, storage=storage, layout=layout)
         engine = serving.PoseFitEngine(registry, max_requests=slots, max_tiles=1 << 19, use_graph=graph)
         engine.run(requests[:2], realtime=False)  # warm-up: code objects, graphs
-        closed = engine.run(requests, realtime=False)
+
+        def progress(done, label=name):
+            if done // 100 > progress.shown:
+                progress.shown = done // 100
+                _log(f"serving {label}: {done} done")
+
+        progress.shown = 0
+        closed = engine.run(requests, realtime=False, progress=progress)
         errors = np.asarray([serving.pose_error(r.pose, truths[int(r.key[3:])]) for r in closed.results])
         row = {"storage": storage, "layout": layout, "slots": slots, "graph": graph,
                "closed_loop": closed.summary(),
@@ -300,8 +314,9 @@ def serving_benchmark(count: int, rates, forms=("batched_bf16", "batched_fp32", 
             row["open_loop"] = {}
             for k, rate in enumerate(rates):
                 at = serving.poisson_arrivals(count, rate, seed=405 + k)
+                progress.shown = 0
                 run = engine.run([dataclasses.replace(q, arrival=float(t)) for q, t in zip(requests, at)],
-                                 realtime=True)
+                                 realtime=True, progress=progress)
                 row["open_loop"][f"{rate:g}_per_s"] = run.summary()
                 _log(f"serving {name} at {rate:g} req/s: p50/p99 {run.summary()['p50_latency_ms']}/"
                      f"{run.summary()['p99_latency_ms']} ms")
@@ -1118,9 +1133,14 @@ def main() -> int:
         return record_counters(args.record_counters, args.calibration_bytes)
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
+    if args.request_only:
+        print(json.dumps(request_benchmark()), flush=True)
+        return 0
     if args.serving_only:
         rates = [float(r) for r in args.serving_rates.split(",") if r]
-        print(json.dumps(serving_benchmark(args.serving_requests, rates, open_forms=("batched_bf16", "plain_fp32"))),
+        forms = tuple(f for f in args.serving_forms.split(",") if f)
+        open_forms = tuple(f for f in forms if f in ("batched_bf16", "plain_fp32"))
+        print(json.dumps(serving_benchmark(args.serving_requests, rates, forms=forms, open_forms=open_forms)),
               flush=True)
         return 0
     if args.kernels_once:

@@ -38,6 +38,8 @@ EXPORTS = (
     "dpt_count_hit_rays",
     "dpt_plan_layout",
     "dpt_valu_probe",
+    "dpt_compose_pose",
+    "dpt_compose_pose_vjp",
     "dpt_last_error",
     "dpt_abi_version",
     "dpt_fit_create",
@@ -217,6 +219,10 @@ def lib():
         h.dpt_plan_layout.restype = ctypes.c_int
         h.dpt_valu_probe.argtypes = [i32, vp, ctypes.POINTER(ctypes.c_double), vp]
         h.dpt_valu_probe.restype = ctypes.c_int
+        h.dpt_compose_pose.argtypes = [vp, vp, i64, vp, vp, vp]
+        h.dpt_compose_pose.restype = ctypes.c_int
+        h.dpt_compose_pose_vjp.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+        h.dpt_compose_pose_vjp.restype = ctypes.c_int
         h.dpt_count_hit_rays.restype = i64
         h.dpt_last_error.argtypes = []
         h.dpt_last_error.restype = ctypes.c_char_p
@@ -388,6 +394,21 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+POSE_JET = 84  # doubles per pose pair in the jet of dpt_compose_pose
+
+
+def compose_pose(anchor: int, tangent: int, count: int, out: int, jet: int, stream: int) -> None:
+    """anchor @ exp(hat(tangent)) for ``count`` pairs (device pointers)."""
+    check(lib().dpt_compose_pose(anchor, tangent, count, out, jet, stream), "dpt_compose_pose")
+
+
+def compose_pose_vjp(anchor: int, jet: int, grad: int, count: int, grad_anchor: int, grad_tangent: int,
+                     stream: int) -> None:
+    """Gradients of the chart from the jet of :func:`compose_pose`."""
+    check(lib().dpt_compose_pose_vjp(anchor, jet, grad, count, grad_anchor, grad_tangent, stream),
+          "dpt_compose_pose_vjp")
 
 
 def valu_probe(kind: int, scratch: int, stream: int) -> float:

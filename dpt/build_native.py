@@ -29,6 +29,7 @@ SOURCES = [
     PACKAGE / "csrc" / "fit.hip",
     PACKAGE / "csrc" / "transport.hip",
     PACKAGE / "csrc" / "probe.hip",
+    PACKAGE / "csrc" / "pose.hip",
 ]
 OUTPUT = PACKAGE / "_lib" / "libdpt_amd.so"
 ARCH = os.environ.get("DPT_AMD_ARCH", "gfx950")

@@ -41,6 +41,9 @@ struct Params {
     int vb;                  // views per workgroup (1, 2 or 4): one wave64 per view when > 1
     int tile_w, tile_h;      // pixels per workgroup tile of one view
     int tiles_x, tiles_y, tiles_per_group, total_tiles, chunk;
+    int segments;            // FP32 VJP: workgroups per tile, each over its own range of samples
+    int base_tiles;          // tiles of one segment (total_tiles / segments)
+    int segment_samples;     // samples per segment (a multiple of chunk_samples)
     int wt_x, wt_y, wt_per_view;  // wave tiles (8x8 pixels, or 64 rays) per view
     const double *panels;    // views x 12
     const double *ray_src;   // views x pixels x 3
@@ -108,8 +111,9 @@ __device__ __forceinline__ bool finite3d(const double *v) {
 __device__ __forceinline__ bool lane_ray(const Params &P, int &view, int &row, int &col, int &wslot,
                                          bool &wave_valid) {
     const int b = blockIdx.x;
-    const int logical = (b % kXcds) * P.chunk + b / kXcds;
+    int logical = (b % kXcds) * P.chunk + b / kXcds;
     if (logical >= P.total_tiles) return false;
+    if (P.segments > 1) logical %= P.base_tiles;  // segment logical / base_tiles
     const int group = logical / P.tiles_per_group;
     const int r = logical - group * P.tiles_per_group;
     const int ty = r / P.tiles_x;

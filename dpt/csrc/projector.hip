@@ -524,6 +524,15 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     bool wave_valid;
     if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
     const double *pose = P.poses + 12 * (size_t)view;
+    // this workgroup's samples [first, last) and its pose partial slot
+    int first = 0, last = P.samples;
+    if (P.segments > 1) {
+        const int b = blockIdx.x;
+        const int seg = ((b % kXcds) * P.chunk + b / kXcds) / P.base_tiles;
+        first = seg * P.segment_samples;
+        last = min(first + P.segment_samples, P.samples);
+        wslot += (view * (P.segments - 1) + seg) * P.wt_per_view;
+    }
 
     // ---- per-ray setup
     bool active = false, summed = false, scattered = false, bad = false;
@@ -587,13 +596,13 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     // Chunks of P.chunk_samples samples; a chunk whose brick overflows LDS is
     // halved (down to kMinChunk samples) until it fits, and only then flushes
     // to device atomics.  Chunk bounds are uniform over the workgroup.
-    for (int k0 = 0, k1 = 0; k0 < P.samples; k0 = k1) {
+    for (int k0 = first, k1 = first; k0 < last; k0 = k1) {
         Brick B{};
         bool use_lds = false;
-        k1 = min(k0 + P.chunk_samples, P.samples);
+        k1 = min(k0 + P.chunk_samples, last);
         if constexpr (VOL) {
             for (int len = P.chunk_samples;; len >>= 1) {
-                k1 = min(k0 + len, P.samples);
+                k1 = min(k0 + len, last);
                 // ---- bound the cells of [k0, k1) over the workgroup
                 int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
                 if (active && summed) {
@@ -747,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
         // for an FP64 one (kernels/projection_fp32.py:212-235), with device
         // atomics (zeros skipped).
         if (scattered) {
-            for (int k = 0; k < P.samples; ++k) {
+            for (int k = first; k < last; ++k) {
                 const float pos = fmaf(sample_centre(P, k), step, iv.lower);
                 const float p0 = fmaf(pos, d[0], o[0]);
                 const float p1 = fmaf(pos, d[1], o[1]);
@@ -1866,6 +1875,7 @@ struct dpt_plan {
     int *d_status = nullptr;
     int32_t *h_status = nullptr;
     double *d_partials = nullptr;
+    int partial_segments = 1;              // d_partials holds this many segments' wave partials
     double *d_acc = nullptr;
     float *d_alpha = nullptr;
     unsigned long long *d_count = nullptr;
@@ -1942,6 +1952,37 @@ void configure_tiles(Params &P, int vb) {
     P.tiles_y = (P.rows + P.tile_h - 1) / P.tile_h;
     P.tiles_per_group = P.tiles_x * P.tiles_y;
     P.total_tiles = P.tiles_per_group * ((P.views + P.vb - 1) / P.vb);
+    P.chunk = (P.total_tiles + kXcds - 1) / kXcds;
+    P.segments = 1;
+    P.base_tiles = P.total_tiles;
+    P.segment_samples = P.samples;
+}
+
+// Sample-range segments of the FP32 VJP: a launch of few tiles (one view of
+// 512^2 is 1024 workgroups, one wave per SIMD) leaves every wave walking all
+// its samples with its memory latency exposed.  Such launches run each tile
+// as `segments` workgroups over consecutive ranges of whole chunks (at least
+// two each), aiming at 16384 waves; the pose partials of a ray's segments are
+// summed in FP64 by the fixed-order reduction, the volume increments of each
+// segment go to the brick as before.  DPT_AMD_VJP_SEGMENTS overrides (1..8).
+void configure_segments(Params &P) {
+    const long long waves = (long long)P.total_tiles * (kBlock / 64);
+    int g = 1;
+    while (g < 8 && waves * (2 * g) <= 16384) g *= 2;
+    if (const char *forced = std::getenv("DPT_AMD_VJP_SEGMENTS")) {
+        const int v = std::atoi(forced);
+        if (v >= 1 && v <= 8) g = v;
+    }
+    const int chunks = (P.samples + P.chunk_samples - 1) / P.chunk_samples;
+    g = std::max(1, std::min(g, chunks / 2));
+    if (g <= 1) return;
+    const int per = (chunks + g - 1) / g;  // whole chunks per segment
+    g = (chunks + per - 1) / per;
+    if (g <= 1) return;
+    P.segments = g;
+    P.segment_samples = per * P.chunk_samples;
+    P.base_tiles = P.total_tiles;
+    P.total_tiles = P.base_tiles * g;
     P.chunk = (P.total_tiles + kXcds - 1) / kXcds;
 }
 
@@ -2437,6 +2478,18 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     Params P = launch_params(p, poses);
     P.volume_f64 = vol64 ? 1 : 0;
     configure_tiles(P, p->vjp_vb);
+    if (p->arithmetic == DPT_ARITH_FP32 && !p->cell_gauss) configure_segments(P);
+    if (pose && P.segments > p->partial_segments) {
+        // grown once per plan, for launches small enough to take segments
+        HIP_TRY(hipStreamSynchronize(s));
+        (void)hipFree(p->d_partials);
+        p->d_partials = nullptr;
+        p->partial_segments = 0;
+        if (hipMalloc((void **)&p->d_partials,
+                      sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * P.views * P.segments) != hipSuccess)
+            return fail(DPT_E_MEMORY, "pose partials allocation failed");
+        p->partial_segments = P.segments;
+    }
     if (!(flags & DPT_NO_CHECKS)) {
         switch (p->storage) {
         case DPT_STORE_F32:
@@ -2484,10 +2537,10 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     if (pose) {
         if (components == 6)
             hipLaunchKernelGGL(pose_reduce_kernel<6>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.wt_per_view, grad_pose, accumulate, p->d_status);
+                               P.segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
         else
             hipLaunchKernelGGL(pose_reduce_kernel<12>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.wt_per_view, grad_pose, accumulate, p->d_status);
+                               P.segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
     }
     if (vol64)
         hipLaunchKernelGGL(finish_volume_kernel<double>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,

@@ -354,11 +354,13 @@ class PoseFitEngine:
             "dpt_fit_step",
         )
 
-    def run(self, requests: list[FitRequest], *, realtime: bool = True) -> RunStats:
+    def run(self, requests: list[FitRequest], *, realtime: bool = True, progress=None) -> RunStats:
         """Serve requests arriving at ``request.arrival`` seconds (continuous batching).
 
         With ``realtime`` the run waits for arrivals on the wall clock; without
         it every request is queued at t = 0 (closed-loop throughput).
+        ``progress``, if given, is called with the number of completed
+        requests each time solves complete.
         """
         import torch
 
@@ -388,6 +390,8 @@ class PoseFitEngine:
                         stop,
                     )
                 )
+            if progress is not None and finished:
+                progress(len(results))
 
         with self._lock, torch.cuda.stream(self.stream):
             while pending or queued or active:

@@ -291,6 +291,21 @@ int64_t dpt_count_hit_rays(dpt_plan *plan, const double *poses, void *stream);
  * counterpart. */
 int dpt_valu_probe(int32_t kind, void *scratch_out, double *rate, void *stream);
 
+/* SE(3) chart G = anchor @ exp(hat(xi)) for `count` pose pairs on
+ * `stream`: anchor 12 FP64 per pair (row-major rotation, translation mm),
+ * tangent 6 FP64 (tx, ty, tz, rx, ry, rz), out 12 FP64; jet receives the
+ * increment exp(hat(xi)) and its six partial derivatives (84 FP64 per
+ * pair) for dpt_compose_pose_vjp.  Replaces the matrix_exp chart of
+ * python/dpt/torch/geometry.py:19-63 (compose_pose) on the device. */
+int dpt_compose_pose(const double *anchor, const double *tangent, int64_t count, double *out, double *jet,
+                     void *stream);
+
+/* Gradients of that chart for a cotangent grad_out (12 FP64 per pair) into
+ * grad_anchor (12 per pair) and grad_tangent (6 per pair), from the jet of
+ * dpt_compose_pose (the backward of python/dpt/torch/geometry.py:19-63). */
+int dpt_compose_pose_vjp(const double *anchor, const double *jet, const double *grad_out, int64_t count,
+                         double *grad_anchor, double *grad_tangent, void *stream);
+
 /* The launch layout a plan chose: out[0] FP64 LDS brick entries of the
  * volume adjoint, out[1] 1 when the FP64 volume adjoint chunks by slabs of
  * grid samples instead of runs of sample indices, out[2] samples per chunk,

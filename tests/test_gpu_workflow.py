@@ -185,3 +185,51 @@ def test_api_project_matches_the_module():
     image = dpt.project(volume, geometry, device="cuda:0", samples_per_ray=samples)
     module = Projector(grid=grid, geometry=geometry, samples_per_ray=samples)
     torch.testing.assert_close(image, module(torch.tensor(values, device="cuda")), rtol=0, atol=0)
+
+
+def _chart_cases():
+    rng = np.random.default_rng(61)
+    tangents = [np.zeros(6), np.full(6, 1e-9), np.array([3.0, -2.0, 1.0, 0.3, -0.2, 0.1]),
+                np.array([120.0, -40.0, 7.0, 2.9, -1.3, 0.4]), np.array([0.0, 0.0, 0.0, 0.0, 0.0, math.pi]),
+                np.array([5.0, 1.0, -2.0, 7.0, -9.0, 11.0])]
+    tangents += [np.concatenate([rng.uniform(-50, 50, 3), rng.uniform(-4, 4, 3)]) for _ in range(10)]
+    return np.stack(tangents)
+
+
+def test_device_pose_chart_matches_matrix_exp_and_its_gradients():
+    """dpt::compose_pose (one HIP launch) against the matrix_exp chart of
+    python/dpt/torch/geometry.py:19-63 on host tensors: values and the
+    gradients of anchor and tangent, for zero, tiny, large-angle (pi and
+    beyond) and random twists, with a broadcast anchor."""
+    from dpt.torch import compose_pose
+
+    tangents = _chart_cases()
+    base = compose_pose(torch.tensor(RigidTransform().packed(), dtype=torch.float64),
+                        torch.tensor([10.0, -20.0, 5.0, 0.4, -0.7, 1.1], dtype=torch.float64))
+    anchors = compose_pose(base.expand(len(tangents), 12),
+                           torch.tensor(np.random.default_rng(62).uniform(-1, 1, (len(tangents), 6))))
+    cot = torch.tensor(np.random.default_rng(63).normal(size=(len(tangents), 12)))
+    for anchor_host in (anchors, base):  # per-pose anchors, one broadcast anchor
+        a_host = anchor_host.clone().requires_grad_(True)
+        x_host = torch.tensor(tangents, requires_grad=True)
+        out_host = compose_pose(a_host, x_host)
+        ga_host, gx_host = torch.autograd.grad(out_host, (a_host, x_host), cot)
+        a_dev = anchor_host.cuda().requires_grad_(True)
+        x_dev = torch.tensor(tangents, device="cuda", requires_grad=True)
+        out_dev = compose_pose(a_dev, x_dev)
+        ga_dev, gx_dev = torch.autograd.grad(out_dev, (a_dev, x_dev), cot.cuda())
+        assert out_dev.shape == out_host.shape
+        torch.testing.assert_close(out_dev.cpu(), out_host.detach(), rtol=1e-13, atol=1e-12)
+        torch.testing.assert_close(ga_dev.cpu(), ga_host, rtol=1e-12, atol=1e-11)
+        torch.testing.assert_close(gx_dev.cpu(), gx_host, rtol=1e-11, atol=1e-10)
+
+
+def test_device_pose_chart_gradcheck():
+    from dpt.torch import compose_pose
+
+    anchor = torch.tensor(RigidTransform().packed(), dtype=torch.float64, device="cuda")
+    anchor = compose_pose(anchor, torch.tensor([1.0, 2.0, 3.0, 0.2, -0.1, 0.3], dtype=torch.float64,
+                                                device="cuda")).detach().requires_grad_(True)
+    tangent = torch.tensor([[0.5, -0.3, 0.2, 0.7, -0.4, 0.25], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]],
+                           dtype=torch.float64, device="cuda", requires_grad=True)
+    assert torch.autograd.gradcheck(compose_pose, (anchor, tangent), eps=1e-6, atol=1e-7, rtol=1e-6)
