@@ -11,8 +11,21 @@ torch.autograd.grad for the field and the FP64 pose together.
 Prints one JSON line.  ``value`` = nominal samples per pass (360*512^2*1024)
 divided by the median iteration time, in Gsamples/s.  ``roofline`` is for the
 dominant kernel, timed with HIP events on its launch stream inside the timed
-region.  ``cpu_baseline`` times the CPU oracle (a restatement of the reference
-FP32 algorithm) on a bounded sample of the same workload.
+region: its measured HBM traffic (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+calibrated on a copy kernel, recorded for a digest of the kernel sources)
+over its launch time against 8 TB/s, and its counted VALU instructions over
+its launch time against the issue rate of independent FMA chains measured in
+the same run.  ``cpu_baseline`` times the CPU oracle (a restatement of the
+reference FP32 algorithm) on a bounded sample of the same workload.
+
+Further blocks of the same line: the reference's component table, its
+full-scan acceptance guards, its single-request case (chest AP view),
+serving (batched BF16 / FP32 cells against the plain per-request form,
+closed and open loop), the precision form, the exact cell-Gauss rule, the
+deterministic gather (full scan and the reference's chest workload) and the
+Monte Carlo transport with its oracle tally check.  ``--*-only`` flags run
+one block alone; ``--kernels-once``, ``--copy-calibration`` and
+``--record-counters`` drive and record the counter runs.
 """
 
 from __future__ import annotations
