@@ -1962,13 +1962,16 @@ void configure_tiles(Params &P, int vb) {
 // 512^2 is 1024 workgroups, one wave per SIMD) leaves every wave walking all
 // its samples with its memory latency exposed.  Such launches run each tile
 // as `segments` workgroups over consecutive ranges of whole chunks (at least
-// two each), aiming at 16384 waves; the pose partials of a ray's segments are
-// summed in FP64 by the fixed-order reduction, the volume increments of each
-// segment go to the brick as before.  DPT_AMD_VJP_SEGMENTS overrides (1..8).
+// two chunks and 256 samples each), aiming at 16384 waves; the pose partials
+// of a ray's segments are summed in FP64 by the fixed-order reduction, the
+// volume increments of each segment go to the brick as before.  A segmented
+// pose gradient is not the reference's one sequential FP32 sum per ray: it
+// is as close to the exact (FP64) value, not bit for bit the FP32 one.
+// DPT_AMD_VJP_SEGMENTS overrides the count (1..8; not the two-chunk floor).
 void configure_segments(Params &P) {
     const long long waves = (long long)P.total_tiles * (kBlock / 64);
     int g = 1;
-    while (g < 8 && waves * (2 * g) <= 16384) g *= 2;
+    while (g < 8 && waves * (2 * g) <= 16384 && P.samples >= 256 * (2 * g)) g *= 2;
     if (const char *forced = std::getenv("DPT_AMD_VJP_SEGMENTS")) {
         const int v = std::atoi(forced);
         if (v >= 1 && v <= 8) g = v;

@@ -163,6 +163,30 @@ def test_fp32_vjp_matches_oracle(name, mode, components):
         np.testing.assert_allclose(p_gpu, p_ref, rtol=1e-4, atol=1e-6 * np.abs(p_ref).max())
 
 
+@pytest.mark.parametrize("segments", [None, "1", "2", "3", "8"])
+@pytest.mark.parametrize("name", ["cone", "workflow"])
+def test_fp32_vjp_sample_segments(name, segments, monkeypatch):
+    """Launches of few tiles split each tile's samples into segments (by
+    default here: 1024 samples, 256 per segment).  The volume gradient keeps
+    the FP32 bar; the pose gradient sums each segment in FP32 and the
+    segments in FP64, so instead of the oracle's one sequential FP32 sum it
+    is checked to be as close to the FP64 VJP as that sum is, and within
+    5e-5 of it."""
+    print(f"case segments {name} {segments}", flush=True)
+    if segments is not None:
+        monkeypatch.setenv("DPT_AMD_VJP_SEGMENTS", segments)
+    grid, values, pose, ref, gpu = _scene(name, 1024)
+    image, _ = ref.forward(values, pose)
+    seeds = np.random.default_rng(29).uniform(-1.0, 1.0, image.size).astype(np.float32)
+    g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose)
+    _, p64, _, _ = ref.vjp(values, seeds.astype(np.float64), pose, precision="fp64", volume=False)
+    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose)
+    assert status.tolist() == [0] * len(status)
+    assert _rel(g_gpu, g_ref) <= 1e-5
+    assert _rel(p_gpu, p64) <= 1.5 * _rel(p_ref, p64) + 1e-7
+    assert _rel(p_gpu, p_ref) <= 5e-5
+
+
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 @pytest.mark.parametrize("cap,chunk", [("512", None), ("512", "7"), (None, "1"), (None, "64"), ("8192", "16")])
 def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch):
