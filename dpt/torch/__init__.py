@@ -1,6 +1,7 @@
-"""Torch adapter: ``Projector``, ``compose_pose`` and the ``dpt::`` operators."""
+"""Torch adapter: ``Projector``, ``compose_pose``, ``PoseObjective`` and the ``dpt::`` operators."""
 
+from .fitting import PoseObjective
 from .geometry import compose_pose
 from .modules import Projector
 
-__all__ = ["Projector", "compose_pose"]
+__all__ = ["PoseObjective", "Projector", "compose_pose"]
