@@ -183,7 +183,8 @@ struct Brick {
 // Add a cell's sums to the LDS brick: eight FP64 LDS atomics (every corner;
 // exact-boundary fractions aside, a visited cell's sums are non-zero).
 __device__ __forceinline__ void flush_lds(const Brick &B, const CellSums &cs, double *brick) {
-    // (z * by + y) * bx + x < ny * nz * nx: operands below 2^24 (grid check)
+    // z * by + y < nz * (ny + 1) <= 2^24 and the index below 2^31 (grid check,
+    // stride padding only where it keeps both)
     const int base =
         (int)(__umul24(__umul24((unsigned)cs.z, (unsigned)B.by) + (unsigned)cs.y, (unsigned)B.bx) + (unsigned)cs.x) -
         B.origin;
@@ -418,7 +419,7 @@ __device__ __forceinline__ int wave_min(int v) {
 // negated maximum over its chunk samples (INT_MAX when it has none).  Every
 // lane gets the same answer (kept in SGPRs); one barrier.
 __device__ __forceinline__ bool chunk_brick(const Params &P, int (*red)[6], int *lo, int *nhi, bool unit_x,
-                                            Brick &B, bool *overflow = nullptr) {
+                                            Brick &B, bool *overflow = nullptr, bool pad = false) {
     const int wv = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -451,8 +452,20 @@ __device__ __forceinline__ bool chunk_brick(const Params &P, int (*red)[6], int 
     B.x0 = __builtin_amdgcn_readfirstlane(l[0]);
     B.y0 = __builtin_amdgcn_readfirstlane(l[1]);
     B.z0 = __builtin_amdgcn_readfirstlane(l[2]);
-    B.bx = __builtin_amdgcn_readfirstlane(e[0]);
-    B.by = __builtin_amdgcn_readfirstlane(e[1]);
+    // pad: odd row and plane strides when the padded box still fits (FP64
+    // entries whose indices differ by a multiple of 32 share an LDS bank, so
+    // even strides line up the corners of neighbouring rays' cells).  Only
+    // for callers that write cells inside the bound box: the padding stays
+    // zero and the drain skips it.
+    int px = e[0], py = e[1];
+    // Padded strides keep flush_lds's global-coordinate index below 2^24 per
+    // operand and 2^31 overall when nz (ny + 1) and nz (ny + 1) (nx + 1) do.
+    const long long plane = (long long)P.nz * (P.ny + 1);
+    if (pad && (long long)(px | 1) * (py | 1) * e[2] <= P.brick_cap && plane <= (1 << 24) &&
+        plane * (P.nx + 1) <= INT_MAX)
+        px |= 1, py |= 1;
+    B.bx = __builtin_amdgcn_readfirstlane(px);
+    B.by = __builtin_amdgcn_readfirstlane(py);
     B.bz = __builtin_amdgcn_readfirstlane(e[2]);
     B.lsx = (unit_x || P.nx > 1) ? 1 : 0;
     B.lsy = P.ny > 1 ? B.bx : 0;
@@ -513,7 +526,14 @@ __device__ __forceinline__ void drain_brick(const Params &P, const Brick &B, dou
 // clamped cell with zero fractions, a zero factor and zero value and slopes:
 // they add nothing, as the reference skips them.
 
-template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT>
+// PIPE (pose only, FP32 storage): the samples of a ray in one software
+// pipeline, the corner loads of sample k + kPipeDepth issued before sample k
+// is summed.  Small launches (one wave per SIMD) expose a memory latency per
+// sample; this hides it without splitting the ray, so each pose sum stays the
+// reference's one sequential FP32 sum.
+constexpr int kPipeDepth = 4;
+
+template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
                                                        const float *__restrict__ seeds,
                                                        double *__restrict__ acc,
@@ -590,9 +610,69 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
     CellSums cs;
     cs.reset();
 
+    static_assert(!PIPE || (POSE && !VOL && !COMPENSATED), "the pipeline is a pose-only FP32 loop");
     if constexpr (VOL) {
         for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
     }
+    if constexpr (PIPE) {
+        if (active) {
+            // slot j holds sample k's corner values, fractions, slopes,
+            // position and support flag from its issue kPipeDepth samples back
+            float ring[kPipeDepth][8], rf[kPipeDepth][6], rpos[kPipeDepth], rlive[kPipeDepth];
+            auto issue = [&](int j, int k) {
+                const float pos = fmaf(sample_centre(P, k), step, iv.lower);
+                const float p0 = fmaf(pos, d[0], o[0]);
+                const float p1 = fmaf(pos, d[1], o[1]);
+                const float p2 = fmaf(pos, d[2], o[2]);
+                Cell32 c;
+                cell32_inside(P, p0, p1, p2, c);  // clamped: the load stays in the field
+                const bool inside = inside32(P, p0, p1, p2);
+                rf[j][0] = inside ? c.fx : 0.0f;
+                rf[j][1] = inside ? c.fy : 0.0f;
+                rf[j][2] = inside ? c.fz : 0.0f;
+                rf[j][3] = inside ? c.slx : 0.0f;
+                rf[j][4] = inside ? c.sly : 0.0f;
+                rf[j][5] = inside ? c.slz : 0.0f;
+                rpos[j] = pos;
+                rlive[j] = inside ? 1.0f : 0.0f;
+                load8<T, LAYOUT>(P, field, c.base, ring[j]);
+            };
+#pragma unroll
+            for (int j = 0; j < kPipeDepth; ++j) issue(j, min(first + j, last - 1));
+            for (int k0 = first; k0 < last; k0 += kPipeDepth) {
+#pragma unroll
+                for (int j = 0; j < kPipeDepth; ++j) {
+                    const int k = k0 + j;
+                    if (k < last) {  // uniform
+                        Cell32 c;
+                        c.fx = rf[j][0];
+                        c.fy = rf[j][1];
+                        c.fz = rf[j][2];
+                        c.slx = rf[j][3];
+                        c.sly = rf[j][4];
+                        c.slz = rf[j][5];
+                        float wt[8], g[3];
+                        weights32(c, wt);
+                        const float value = interpolate32(wt, ring[j]) * rlive[j];
+                        slopes32(c, ring[j], g);
+                        const float pos = rpos[j];
+                        mean.add(value);
+                        dor0.add(g[0]);
+                        dor1.add(g[1]);
+                        dor2.add(g[2]);
+                        dd0.fma_add(pos, g[0]);
+                        dd1.fma_add(pos, g[1]);
+                        dd2.fma_add(pos, g[2]);
+                        const float alpha = sample_alpha(P, k);
+                        const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
+                        shl.fma_add(1.0f - alpha, along);
+                        shu.fma_add(alpha, along);
+                    }
+                    if (k + kPipeDepth < last) issue(j, k + kPipeDepth);  // uniform
+                }
+            }
+        }
+    } else {
     // Chunks of P.chunk_samples samples; a chunk whose brick overflows LDS is
     // halved (down to kMinChunk samples) until it fits, and only then flushes
     // to device atomics.  Chunk bounds are uniform over the workgroup.
@@ -621,7 +701,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 // compile-time unit x stride lets the LDS atomics of
                 // x-adjacent corners use the instruction's offset field
                 bool overflow = false;
-                use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B, &overflow);
+                use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B, &overflow, true);
                 if (!overflow || len <= kMinChunk) break;
                 __syncthreads();  // the next bound rewrites bound_red
             }
@@ -721,6 +801,7 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             }
             __syncthreads();
         }
+    }
     }
     if constexpr (POSE) {
         double gradient[kMaxComponents];
@@ -1965,8 +2046,9 @@ void configure_tiles(Params &P, int vb) {
 // two chunks and 256 samples each), aiming at 16384 waves; the pose partials
 // of a ray's segments are summed in FP64 by the fixed-order reduction, the
 // volume increments of each segment go to the brick as before.  A segmented
-// pose gradient is not the reference's one sequential FP32 sum per ray: it
-// is as close to the exact (FP64) value, not bit for bit the FP32 one.
+// pose gradient is not the reference's one sequential FP32 sum per ray, so
+// only 16-bit storage (blocked compensated sums, no FP32 counterpart)
+// segments the pose; FP32 storage runs it unsegmented in the pipelined loop.
 // DPT_AMD_VJP_SEGMENTS overrides the count (1..8; not the two-chunk floor).
 void configure_segments(Params &P) {
     const long long waves = (long long)P.total_tiles * (kBlock / 64);
@@ -2070,11 +2152,19 @@ double *pose_destination(dpt_plan *p) {
 }
 
 template <typename T, int C, int LAYOUT>
-void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, hipStream_t s) {
+void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, bool vol, bool pose, bool pipe,
+                  hipStream_t s) {
     constexpr bool HALF = !std::is_same<T, float>::value;
     const dim3 grid(tile_grid(P)), block(kBlock);
     const size_t lds = (size_t)P.brick_cap * sizeof(double);
     double *partials = pose_destination<C>(p);
+    if constexpr (!HALF && (C == 6 || C == 12)) {
+        if (pipe && pose && !vol) {
+            hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, false, LAYOUT, true>), grid, block, 0, s, P, f, sd,
+                               p->d_acc, partials);
+            return;
+        }
+    }
     if (vol && pose)
         hipLaunchKernelGGL((vjp32_kernel<T, true, true, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd, p->d_acc,
                            partials);
@@ -2088,18 +2178,18 @@ void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, boo
 
 template <typename T, int C>
 int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
-                 bool cells, hipStream_t s) {
+                 bool cells, hipStream_t s, bool pipe = false) {
     const T *f = static_cast<const T *>(field);
     const dim3 grid(tile_grid(P)), block(kBlock);
     if (p->arithmetic == DPT_ARITH_FP32) {
         const float *sd = static_cast<const float *>(seeds);
         if (cells && pose && P.nx > 1)
             launch_vjp32<T, C, kLayoutCells>(p, cells_params(p, P), static_cast<const T *>(p->d_cells), sd, vol,
-                                             pose, s);
+                                             pose, pipe, s);
         else if (P.nx > 1)
-            launch_vjp32<T, C, kLayoutPair>(p, P, f, sd, vol, pose, s);
+            launch_vjp32<T, C, kLayoutPair>(p, P, f, sd, vol, pose, pipe, s);
         else
-            launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, s);
+            launch_vjp32<T, C, kLayoutScalar>(p, P, f, sd, vol, pose, pipe, s);
     } else if (p->cell_gauss) {
         const double *sd = static_cast<const double *>(seeds);
         const size_t lds = (size_t)P.brick_cap * sizeof(double);
@@ -2139,11 +2229,11 @@ int launch_vjp_c(dpt_plan *p, const Params &P, const void *field, const void *se
 
 template <typename T>
 int launch_vjp(dpt_plan *p, const Params &P, const void *field, const void *seeds, bool vol, bool pose,
-               int components, bool cells, hipStream_t s) {
+               int components, bool cells, hipStream_t s, bool pipe = false) {
     if (components == kRayCoordinates) return launch_vjp_c<T, kRayCoordinates>(p, P, field, seeds, false, true, cells, s);
     if (components == kRayTangent) return launch_vjp_c<T, kRayTangent>(p, P, field, seeds, false, true, cells, s);
-    if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, cells, s);
-    return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, cells, s);
+    if (components == 6) return launch_vjp_c<T, 6>(p, P, field, seeds, vol, pose, cells, s, pipe);
+    return launch_vjp_c<T, 12>(p, P, field, seeds, vol, pose, cells, s, pipe);
 }
 
 int launch_cells(const void *src, void *dst, int nx, int ny, int nz, int storage, hipStream_t s) {
@@ -2481,17 +2571,23 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     Params P = launch_params(p, poses);
     P.volume_f64 = vol64 ? 1 : 0;
     configure_tiles(P, p->vjp_vb);
+    const Params whole = P;
     if (p->arithmetic == DPT_ARITH_FP32 && !p->cell_gauss) configure_segments(P);
-    if (pose && P.segments > p->partial_segments) {
+    // FP32 storage keeps each ray's pose sums one sequential FP32 sum, as the
+    // reference: a segmented launch takes the pose unsegmented through the
+    // pipelined loop, beside a segmented volume-only launch
+    const bool pipe = pose && P.segments > 1 && p->storage == DPT_STORE_F32;
+    const int pose_segments = pipe ? 1 : P.segments;
+    if (pose && pose_segments > p->partial_segments) {
         // grown once per plan, for launches small enough to take segments
         HIP_TRY(hipStreamSynchronize(s));
         (void)hipFree(p->d_partials);
         p->d_partials = nullptr;
         p->partial_segments = 0;
         if (hipMalloc((void **)&p->d_partials,
-                      sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * P.views * P.segments) != hipSuccess)
+                      sizeof(double) * kMaxComponents * (size_t)P.wt_per_view * P.views * pose_segments) != hipSuccess)
             return fail(DPT_E_MEMORY, "pose partials allocation failed");
-        p->partial_segments = P.segments;
+        p->partial_segments = pose_segments;
     }
     if (!(flags & DPT_NO_CHECKS)) {
         switch (p->storage) {
@@ -2527,8 +2623,11 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
         if (rc != DPT_OK) return rc;
     }
     timer_begin(p, 1, s);
-    int rc;
-    if (p->storage == DPT_STORE_F32)
+    int rc = DPT_OK;
+    if (pipe) {
+        if (vol) rc = launch_vjp<float>(p, P, field, cotangent, true, false, components, false, s);
+        if (rc == DPT_OK) rc = launch_vjp<float>(p, whole, field, cotangent, false, true, components, cells, s, true);
+    } else if (p->storage == DPT_STORE_F32)
         rc = launch_vjp<float>(p, P, field, cotangent, vol, pose, components, cells, s);
     else if (p->storage == DPT_STORE_BF16)
         rc = launch_vjp<__hip_bfloat16>(p, P, field, cotangent, vol, pose, components, cells, s);
@@ -2540,10 +2639,10 @@ int dpt_vjp(dpt_plan *p, const void *field, const double *poses, const void *cot
     if (pose) {
         if (components == 6)
             hipLaunchKernelGGL(pose_reduce_kernel<6>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
+                               pose_segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
         else
             hipLaunchKernelGGL(pose_reduce_kernel<12>, dim3(P.views), dim3(kBlock), 0, s, p->d_partials,
-                               P.segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
+                               pose_segments * P.wt_per_view, grad_pose, accumulate, p->d_status);
     }
     if (vol64)
         hipLaunchKernelGGL(finish_volume_kernel<double>, dim3(grid_blocks(p->voxels)), dim3(kBlock), 0, s,

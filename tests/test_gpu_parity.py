@@ -107,17 +107,19 @@ SCENES = {
 }
 
 
-def _scene(name, samples, precision="float32"):
+def _scene(name, samples, precision="float32", **native):
     if name == "oblique":
         grid, src, end, pose, values = scenes.oblique_scene()
-        return grid, values, pose, O.Scene(grid, samples, rays=(src, end)), NativeScene(grid, samples, rays=(src, end), precision=precision)
+        return (grid, values, pose, O.Scene(grid, samples, rays=(src, end)),
+                NativeScene(grid, samples, rays=(src, end), precision=precision, **native))
     if name == "workflow":
         grid, panels, values = scenes.workflow_scene()
         pose = scenes.compose(scenes.IDENTITY12, (0.03, -0.02, 0.01, 0.04, -0.03, 0.02))
     else:
         grid, panels, values = _cone()
         pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
-    return grid, values, pose, O.Scene(grid, samples, panels=panels), NativeScene(grid, samples, panels=panels, precision=precision)
+    return (grid, values, pose, O.Scene(grid, samples, panels=panels),
+            NativeScene(grid, samples, panels=panels, precision=precision, **native))
 
 
 @pytest.mark.parametrize("samples", [17, 97, 256])
@@ -165,26 +167,50 @@ def test_fp32_vjp_matches_oracle(name, mode, components):
 
 @pytest.mark.parametrize("segments", [None, "1", "2", "3", "8"])
 @pytest.mark.parametrize("name", ["cone", "workflow"])
-def test_fp32_vjp_sample_segments(name, segments, monkeypatch):
+@pytest.mark.parametrize("cells", [False, True])
+def test_fp32_vjp_sample_segments(name, segments, cells, monkeypatch):
     """Launches of few tiles split each tile's samples into segments (by
-    default here: 1024 samples, 256 per segment).  The volume gradient keeps
-    the FP32 bar; the pose gradient sums each segment in FP32 and the
-    segments in FP64, so instead of the oracle's one sequential FP32 sum it
-    is checked to be as close to the FP64 VJP as that sum is, and within
-    5e-5 of it."""
-    print(f"case segments {name} {segments}", flush=True)
+    default here: 1024 samples, 256 per segment) for the volume adjoint; the
+    pose adjoint of FP32 storage runs those launches unsegmented through the
+    pipelined sample loop, so each ray's pose sums are the oracle's one
+    sequential FP32 sum: the pose gradient agrees up to the FP64 order of the
+    cross-ray sums, joint and pose-only, linear and cell gathers."""
+    print(f"case segments {name} {segments} cells={cells}", flush=True)
     if segments is not None:
         monkeypatch.setenv("DPT_AMD_VJP_SEGMENTS", segments)
-    grid, values, pose, ref, gpu = _scene(name, 1024)
+    grid, values, pose, ref, gpu = _scene(name, 1024, extra_needs=_native.GATHER_CELLS if cells else 0)
     image, _ = ref.forward(values, pose)
     seeds = np.random.default_rng(29).uniform(-1.0, 1.0, image.size).astype(np.float32)
     g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose)
-    _, p64, _, _ = ref.vjp(values, seeds.astype(np.float64), pose, precision="fp64", volume=False)
-    g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose)
-    assert status.tolist() == [0] * len(status)
-    assert _rel(g_gpu, g_ref) <= 1e-5
-    assert _rel(p_gpu, p64) <= 1.5 * _rel(p_ref, p64) + 1e-7
-    assert _rel(p_gpu, p_ref) <= 5e-5
+    for vol in (True, False):
+        g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose, volume=vol)
+        assert status.tolist() == [0] * len(status)
+        if vol:
+            assert _rel(g_gpu, g_ref) <= 1e-5
+        assert _rel(p_gpu, p_ref) <= 1e-12, (vol, _rel(p_gpu, p_ref))
+
+
+def test_half_storage_pose_segments_stay_near_the_fp64_vjp(monkeypatch):
+    """16-bit storage segments the pose too (its blocked compensated sums have
+    no sequential FP32 counterpart): against the FP64 VJP of the rounded field
+    it stays as close as the unsegmented launch, and as the oracle's FP32 VJP
+    of that field."""
+    print("case half segments", flush=True)
+    grid, values, pose, ref, _ = _scene("cone", 1024, storage="bfloat16")
+    rounded = torch.tensor(values, dtype=torch.float32).to(torch.bfloat16).float().numpy()
+    image, _ = ref.forward(rounded, pose)
+    seeds = np.random.default_rng(31).uniform(-1.0, 1.0, image.size).astype(np.float32)
+    _, p64, _, _ = ref.vjp(rounded, seeds.astype(np.float64), pose, precision="fp64", volume=False)
+    _, p32, _, _ = ref.vjp(rounded, seeds, pose, volume=False)
+    errors = []
+    for segments in ("1", "4"):
+        monkeypatch.setenv("DPT_AMD_VJP_SEGMENTS", segments)
+        *_, gpu = _scene("cone", 1024, storage="bfloat16")
+        _, p_gpu, status = gpu.vjp(values, seeds, pose, volume=False)
+        assert status.tolist() == [0] * len(status)
+        errors.append(_rel(p_gpu, p64))
+    assert errors[1] <= 1.5 * errors[0] + 1e-7, errors
+    assert max(errors) <= 1.5 * _rel(p32, p64) + 1e-7, (errors, _rel(p32, p64))
 
 
 @pytest.mark.parametrize("precision", ["float32", "float64"])
