@@ -528,10 +528,14 @@ __device__ __forceinline__ void drain_brick(const Params &P, const Brick &B, dou
 
 // PIPE (pose only, FP32 storage): the samples of a ray in one software
 // pipeline, the corner loads of sample k + kPipeDepth issued before sample k
-// is summed.  Small launches (one wave per SIMD) expose a memory latency per
-// sample; this hides it without splitting the ray, so each pose sum stays the
-// reference's one sequential FP32 sum.
-constexpr int kPipeDepth = 4;
+// is summed.  Small launches (one wave per SIMD or fewer) expose a memory
+// latency per sample; this hides it without splitting the ray, so each pose
+// sum stays the reference's one sequential FP32 sum.  Eight slots of corner
+// values, fractions and positions take 220 VGPRs (two waves per SIMD); on a
+// one-view 512^2 launch depth 8 ran the pose VJP in 0.61 ms (central-window
+// cotangent) and 0.84 ms (every ray), depth 4 in 0.86 / 1.31 ms, and
+// recomputing the cell at the sum instead of keeping it was slower.
+constexpr int kPipeDepth = 8;
 
 template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
@@ -637,40 +641,47 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 rlive[j] = inside ? 1.0f : 0.0f;
                 load8<T, LAYOUT>(P, field, c.base, ring[j]);
             };
+            auto consume = [&](int j, int k) {
+                Cell32 c;
+                c.fx = rf[j][0];
+                c.fy = rf[j][1];
+                c.fz = rf[j][2];
+                c.slx = rf[j][3];
+                c.sly = rf[j][4];
+                c.slz = rf[j][5];
+                float wt[8], g[3];
+                weights32(c, wt);
+                const float value = interpolate32(wt, ring[j]) * rlive[j];
+                slopes32(c, ring[j], g);
+                const float pos = rpos[j];
+                mean.add(value);
+                dor0.add(g[0]);
+                dor1.add(g[1]);
+                dor2.add(g[2]);
+                dd0.fma_add(pos, g[0]);
+                dd1.fma_add(pos, g[1]);
+                dd2.fma_add(pos, g[2]);
+                const float alpha = sample_alpha(P, k);
+                const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
+                shl.fma_add(1.0f - alpha, along);
+                shu.fma_add(alpha, along);
+            };
 #pragma unroll
             for (int j = 0; j < kPipeDepth; ++j) issue(j, min(first + j, last - 1));
-            for (int k0 = first; k0 < last; k0 += kPipeDepth) {
+            // whole groups without branches, so that the loads stay in flight
+            // across iterations; the last group's issues repeat sample
+            // last - 1 and are not consumed
+            int k0 = first;
+            for (; k0 + kPipeDepth <= last; k0 += kPipeDepth) {
 #pragma unroll
                 for (int j = 0; j < kPipeDepth; ++j) {
-                    const int k = k0 + j;
-                    if (k < last) {  // uniform
-                        Cell32 c;
-                        c.fx = rf[j][0];
-                        c.fy = rf[j][1];
-                        c.fz = rf[j][2];
-                        c.slx = rf[j][3];
-                        c.sly = rf[j][4];
-                        c.slz = rf[j][5];
-                        float wt[8], g[3];
-                        weights32(c, wt);
-                        const float value = interpolate32(wt, ring[j]) * rlive[j];
-                        slopes32(c, ring[j], g);
-                        const float pos = rpos[j];
-                        mean.add(value);
-                        dor0.add(g[0]);
-                        dor1.add(g[1]);
-                        dor2.add(g[2]);
-                        dd0.fma_add(pos, g[0]);
-                        dd1.fma_add(pos, g[1]);
-                        dd2.fma_add(pos, g[2]);
-                        const float alpha = sample_alpha(P, k);
-                        const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
-                        shl.fma_add(1.0f - alpha, along);
-                        shu.fma_add(alpha, along);
-                    }
-                    if (k + kPipeDepth < last) issue(j, k + kPipeDepth);  // uniform
+                    consume(j, k0 + j);
+                    issue(j, min(k0 + j + kPipeDepth, last - 1));
                 }
             }
+#pragma unroll
+            for (int j = 0; j < kPipeDepth; ++j)
+                if (k0 + j < last) consume(j, k0 + j);  // uniform
         }
     } else {
     // Chunks of P.chunk_samples samples; a chunk whose brick overflows LDS is
