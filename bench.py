@@ -77,6 +77,7 @@ def _args():
     parser.add_argument("--no-guards", action="store_true", help="skip the full-scan acceptance guards")
     parser.add_argument("--no-request", action="store_true", help="skip the single-request (chest AP) timings")
     parser.add_argument("--request-only", action="store_true", help="only the single-request (chest AP) timings")
+    parser.add_argument("--request-repeats", type=int, default=300, help="timed repeats per single-request figure")
     parser.add_argument("--record-counters", nargs="+", default=None, metavar="CSV",
                         help="instead of the benchmark, write profiles/projector_counters.json from rocprofv3 "
                              "counter CSVs of --kernels-once runs (copy-calibration CSVs first, see "
@@ -1147,7 +1148,7 @@ def main() -> int:
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
     if args.request_only:
-        print(json.dumps(request_benchmark()), flush=True)
+        print(json.dumps(request_benchmark(args.request_repeats)), flush=True)
         return 0
     if args.serving_only:
         rates = [float(r) for r in args.serving_rates.split(",") if r]
@@ -1322,7 +1323,7 @@ def main() -> int:
         result["guards"] = fullscan_guards(grid, geometry, values, args.samples)
     if not args.no_request:
         _log("single request (chest AP)")
-        result["request"] = request_benchmark()
+        result["request"] = request_benchmark(args.request_repeats)
     if not args.no_serving:
         _log("serving benchmark")
         rates = [float(r) for r in args.serving_rates.split(",") if r]

@@ -51,10 +51,22 @@
 
 namespace dpt {
 
+// PIPE (FP32 storage; forward and pose-only VJP): the samples of a ray in one
+// software pipeline, the corner loads of sample k + kPipeDepth issued before sample k
+// is summed.  Small launches (one wave per SIMD or fewer) expose a memory
+// latency per sample; this hides it without splitting the ray, so each pose
+// sum stays the reference's one sequential FP32 sum (forward: the same
+// sequential sum of interpolated values).  Eight slots of corner
+// values, fractions and positions take 220 VGPRs (two waves per SIMD); on a
+// one-view 512^2 launch depth 8 ran the pose VJP in 0.61 ms (central-window
+// cotangent) and 0.84 ms (every ray), depth 4 in 0.86 / 1.31 ms, and
+// recomputing the cell at the sum instead of keeping it was slower.
+constexpr int kPipeDepth = 8;
+
 // ------------------------------------------------------- FP32 forward ------
 // kernels/projection_fp32.py:334-374
 
-template <typename T, bool COMPENSATED, int LAYOUT>
+template <typename T, bool COMPENSATED, int LAYOUT, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__restrict__ field,
                                                            float *__restrict__ out) {
     int view, row, col, wslot;
@@ -109,12 +121,65 @@ __global__ __launch_bounds__(kBlock) void forward32_kernel(Params P, const T *__
                 total.add(interpolate32(wt, v));
             }
         };
-        // per-sample support tests only for waves with a ray whose end
-        // samples leave the support
-        if (__builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, 0, P.samples)) == 0)
+        static_assert(!PIPE || !COMPENSATED, "the pipeline is an FP32-storage loop");
+        if constexpr (PIPE) {
+            // slot j: corner values, fractions and support flag of the sample
+            // issued kPipeDepth samples back; an outside sample adds -0, the
+            // identity of FP32 addition (the reference skips it)
+            float ring[kPipeDepth][8], rf[kPipeDepth][3], cur[8];
+            bool rin[kPipeDepth], fresh[kPipeDepth];
+            int prev = -1;
+            auto issue = [&](int j, int k) {
+                const float pos = fmaf(sample_centre(P, k), step, iv.lower);
+                const float p0 = fmaf(pos, d[0], o[0]);
+                const float p1 = fmaf(pos, d[1], o[1]);
+                const float p2 = fmaf(pos, d[2], o[2]);
+                Cell32 c;
+                cell32_inside(P, p0, p1, p2, c);  // clamped: the load stays in the field
+                rin[j] = inside32(P, p0, p1, p2);
+                rf[j][0] = c.fx;
+                rf[j][1] = c.fy;
+                rf[j][2] = c.fz;
+                // only lanes whose cell changed since the previous sample load
+                fresh[j] = c.base != prev;
+                prev = c.base;
+                if (fresh[j]) load8<T, LAYOUT>(P, field, c.base, ring[j]);
+            };
+            auto consume = [&](int j) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cur[q] = fresh[j] ? ring[j][q] : cur[q];
+                Cell32 c;
+                c.fx = rf[j][0];
+                c.fy = rf[j][1];
+                c.fz = rf[j][2];
+                float wt[8];
+                weights32(c, wt);
+                const float value = interpolate32(wt, cur);
+                total.add(rin[j] ? value : -0.0f);
+            };
+#pragma unroll
+            for (int q = 0; q < 8; ++q) cur[q] = 0.0f;
+            const int last = P.samples;
+#pragma unroll
+            for (int j = 0; j < kPipeDepth; ++j) issue(j, min(j, last - 1));
+            int k0 = 0;
+            for (; k0 + kPipeDepth <= last; k0 += kPipeDepth) {
+#pragma unroll
+                for (int j = 0; j < kPipeDepth; ++j) {
+                    consume(j);
+                    issue(j, min(k0 + j + kPipeDepth, last - 1));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kPipeDepth; ++j)
+                if (k0 + j < last) consume(j);  // uniform
+        } else if (__builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, 0, P.samples)) == 0) {
+            // per-sample support tests only for waves with a ray whose end
+            // samples leave the support
             walk(std::false_type{});
-        else
+        } else {
             walk(std::true_type{});
+        }
         if (bad) flag(P, view, DPT_STATUS_RANGE);
         result = total.value() * (len * step);
     }
@@ -526,16 +591,6 @@ __device__ __forceinline__ void drain_brick(const Params &P, const Brick &B, dou
 // clamped cell with zero fractions, a zero factor and zero value and slopes:
 // they add nothing, as the reference skips them.
 
-// PIPE (pose only, FP32 storage): the samples of a ray in one software
-// pipeline, the corner loads of sample k + kPipeDepth issued before sample k
-// is summed.  Small launches (one wave per SIMD or fewer) expose a memory
-// latency per sample; this hides it without splitting the ray, so each pose
-// sum stays the reference's one sequential FP32 sum.  Eight slots of corner
-// values, fractions and positions take 220 VGPRs (two waves per SIMD); on a
-// one-view 512^2 launch depth 8 ran the pose VJP in 0.61 ms (central-window
-// cotangent) and 0.84 ms (every ray), depth 4 in 0.86 / 1.31 ms, and
-// recomputing the cell at the sum instead of keeping it was slower.
-constexpr int kPipeDepth = 8;
 
 template <typename T, bool VOL, bool POSE, int C, bool COMPENSATED, int LAYOUT, bool PIPE = false>
 __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__restrict__ field,
@@ -865,6 +920,225 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                     bad |= !isfinite(inc);
                     if (inc != 0.0f) unsafeAtomicAdd(&acc[voxel + corner_offset(P, kk)], (double)inc);
                 }
+            }
+        }
+    }
+    if (bad) flag(P, view, DPT_STATUS_RANGE);
+}
+
+// FP32 volume adjoint by slab chunks (plans whose index chunks overflow the
+// brick: tiles at the grid's silhouette, where the clipped intervals of a
+// tile's rays differ in length and their k-th samples spread apart).  The
+// workgroup cuts its samples [first, last) into slabs of W grid samples
+// along its dominant axis A and each lane visits, per chunk, its samples
+// whose A coordinate lies in the slab (W: the advance of the fastest lane
+// over the plan's chunk length).  Boundaries are integers and both sides of
+// one use the same expression, so every sample falls in exactly one chunk;
+// the brick bound comes from the chunk's own end samples, so any rounding in
+// the inversion only moves samples between neighbouring chunks.  Per-sample
+// arithmetic, cell sums and flushes are those of vjp32_kernel's volume part.
+__global__ __launch_bounds__(kBlock) void vol32_slab_kernel(Params P, const float *__restrict__ seeds,
+                                                            double *__restrict__ acc) {
+    extern __shared__ double brick[];
+    __shared__ int bound_red[kBlock / 64][6];
+    int view, row, col, wslot;
+    bool wave_valid;
+    if (!lane_ray(P, view, row, col, wslot, wave_valid)) return;
+    const double *pose = P.poses + 12 * (size_t)view;
+    int first = 0, last = P.samples;
+    if (P.segments > 1) {
+        const int b = blockIdx.x;
+        const int seg = ((b % kXcds) * P.chunk + b / kXcds) / P.base_tiles;
+        first = seg * P.segment_samples;
+        last = min(first + P.segment_samples, P.samples);
+    }
+    bool active = false, summed = false, scattered = false, bad = false;
+    float o[3] = {0, 0, 0}, d[3] = {0, 0, 0}, step = 0.0f, volume_factor = 0.0f;
+    Interval32 iv{};
+    if (ray_in_range(P, view, row, col)) {
+        const size_t ray = ((size_t)view * P.rows + row) * P.cols + col;
+        World w;
+        ray_world(P, pose, view, row, col, w);
+        Ray r;
+        ray_grid(P, w, r);
+        o[0] = (float)r.o[0];
+        o[1] = (float)r.o[1];
+        o[2] = (float)r.o[2];
+        d[0] = (float)r.d[0];
+        d[1] = (float)r.d[1];
+        d[2] = (float)r.d[2];
+        const float len = (float)r.len;
+        if (!r.valid || !finite3f(o[0], o[1], o[2]) || !finite3f(d[0], d[1], d[2]) || !isfinite(len) ||
+            len <= 0.0f) {
+            flag(P, view, DPT_STATUS_RANGE);
+        } else {
+            iv = interval32(P, o, d);
+            const float span = iv.upper - iv.lower;
+            const float seed = seeds[ray];
+            if (!isfinite(seed)) flag(P, view, DPT_STATUS_RANGE);
+            if (span > 0.0f && seed != 0.0f) {
+                active = true;
+                step = span / (float)P.samples;
+                volume_factor = seed * (len * step);
+                const bool finite = finite_span(o, d, iv.lower, step, 0, P.samples);
+                bad = !finite;
+                const float m = fabsf(volume_factor);
+                summed = finite && m >= 0x1p-102f && m <= 0x1p96f;
+                scattered = !summed;
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < P.brick_cap; i += kBlock) brick[i] = 0.0;
+
+    // ---- slabs of the workgroup's in-loop rays over [first, last)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
+    const int mine = ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2);
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) bound_red[wv][a] = __popcll(__builtin_amdgcn_ballot_w64(summed && mine == a));
+    __syncthreads();
+    int votes[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) votes[a] = bound_red[0][a] + bound_red[1][a] + bound_red[2][a] + bound_red[3][a];
+    const int axis = votes[0] >= votes[1] && votes[0] >= votes[2] ? 0 : (votes[1] >= votes[2] ? 1 : 2);
+    const int n_axis = axis == 0 ? P.nx : (axis == 1 ? P.ny : P.nz);
+    const float oa = axis == 0 ? o[0] : (axis == 1 ? o[1] : o[2]);
+    const float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+    int lo = INT_MAX, nhi = INT_MAX, nadv = INT_MAX;
+    if (summed) {
+        const float q0 = fmaf(fmaf((float)first + 0.5f, step, iv.lower), da, oa);
+        const float q1 = fmaf(fmaf((float)(last - 1) + 0.5f, step, iv.lower), da, oa);
+        lo = (int)floorf(fminf(fmaxf(fminf(q0, q1), -2.0f), (float)n_axis + 1.0f));
+        nhi = -(int)ceilf(fminf(fmaxf(fmaxf(q0, q1), -2.0f), (float)n_axis + 1.0f));
+        // advance per sample in 1/64 grid samples, at least 1
+        nadv = -(int)fminf(fmaxf(ceilf(step * fabsf(da) * 64.0f), 1.0f), 1048576.0f);
+    }
+    lo = wave_min(lo);
+    nhi = wave_min(nhi);
+    nadv = wave_min(nadv);
+    __syncthreads();  // every wave has read the votes
+    if (lane == 0) {
+        bound_red[wv][3] = lo;
+        bound_red[wv][4] = nhi;
+        bound_red[wv][5] = nadv;
+    }
+    __syncthreads();
+    int slab_lo = min(min(bound_red[0][3], bound_red[1][3]), min(bound_red[2][3], bound_red[3][3]));
+    const int slab_hi = -min(min(bound_red[0][4], bound_red[1][4]), min(bound_red[2][4], bound_red[3][4]));
+    const int adv = -min(min(bound_red[0][5], bound_red[1][5]), min(bound_red[2][5], bound_red[3][5]));
+    __syncthreads();  // bound_red is reused by the chunk bounds
+    int width = 1, count = 1;
+    if (slab_lo == INT_MAX) {
+        slab_lo = 0;
+    } else {
+        width = max(1, (P.chunk_samples * adv + 63) / 64);
+        count = max(1, (slab_hi - slab_lo + width - 1) / width);
+    }
+
+    CellSums cs;
+    cs.reset();
+    for (int ch = 0; ch < count; ++ch) {
+        // this lane's samples [k0, k1) in slab ch
+        int k0 = first, k1 = first;
+        if (summed) {
+            if (da == 0.0f) {
+                const int own = min(max((int)floorf((oa - (float)slab_lo) / (float)width), 0), count - 1);
+                k1 = ch == own ? last : first;
+            } else {
+                // t(b): the sample index where the coordinate crosses b
+                const int js = da > 0.0f ? ch : count - 1 - ch;
+                const float b0 = (float)(slab_lo + js * width), b1 = (float)(slab_lo + (js + 1) * width);
+                const float t0 = ((b0 - oa) / da - iv.lower) / step - 0.5f;
+                const float t1 = ((b1 - oa) / da - iv.lower) / step - 0.5f;
+                const float lo_k = (float)first, hi_k = (float)last;
+                if (da > 0.0f) {
+                    k0 = ch == 0 ? first : (int)fminf(fmaxf(ceilf(t0), lo_k), hi_k);
+                    k1 = ch == count - 1 ? last : (int)fminf(fmaxf(ceilf(t1), lo_k), hi_k);
+                } else {
+                    k0 = ch == 0 ? first : (int)fminf(fmaxf(floorf(t1) + 1.0f, lo_k), hi_k);
+                    k1 = ch == count - 1 ? last : (int)fminf(fmaxf(floorf(t0) + 1.0f, lo_k), hi_k);
+                }
+                k1 = max(k1, k0);
+            }
+        }
+        int blo[3] = {INT_MAX, INT_MAX, INT_MAX}, bnhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+        if (k1 > k0) {
+            const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
+            const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
+            int la[3], lb[3];
+            lower_cell32(P, fmaf(pa, d[0], o[0]), fmaf(pa, d[1], o[1]), fmaf(pa, d[2], o[2]), la);
+            lower_cell32(P, fmaf(pb, d[0], o[0]), fmaf(pb, d[1], o[1]), fmaf(pb, d[2], o[2]), lb);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                blo[a] = min(la[a], lb[a]);
+                bnhi[a] = -max(la[a], lb[a]);
+            }
+        }
+        Brick B{};
+        const bool use_lds = chunk_brick(P, bound_red, blo, bnhi, P.nx > 1, B, nullptr, true);
+        int cur = -1;
+        for (int k = k0; k < k1; ++k) {
+            const float pos = fmaf((float)k + 0.5f, step, iv.lower);
+            const float p0 = fmaf(pos, d[0], o[0]);
+            const float p1 = fmaf(pos, d[1], o[1]);
+            const float p2 = fmaf(pos, d[2], o[2]);
+            Cell32 c;
+            cell32_inside(P, p0, p1, p2, c);
+            const bool inside = inside32(P, p0, p1, p2);
+            // an outside sample adds nothing (zero fractions and factor)
+            c.fx = inside ? c.fx : 0.0f;
+            c.fy = inside ? c.fy : 0.0f;
+            c.fz = inside ? c.fz : 0.0f;
+            const float factor = inside ? volume_factor : 0.0f;
+            if (c.base != cur) {
+                if (cs.x >= 0) {
+                    if (use_lds)
+                        flush_lds(B, cs, brick);
+                    else
+                        flush_device(P, cs, acc);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) cs.s[kk] = 0.0f;
+                cs.x = c.lx;
+                cs.y = c.ly;
+                cs.z = c.lz;
+                cur = c.base;
+            }
+            float wt[8];
+            weights32(c, wt);
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) cs.s[kk] = fmaf(wt[kk], factor, cs.s[kk]);
+        }
+        if (cs.x >= 0) {
+            if (use_lds)
+                flush_lds(B, cs, brick);
+            else
+                flush_device(P, cs, acc);
+        }
+        cs.reset();
+        __syncthreads();
+        if (use_lds) drain_brick(P, B, brick, acc);
+        __syncthreads();
+    }
+    // rays outside the in-loop factor range: per-increment device atomics,
+    // as vjp32_kernel
+    if (scattered) {
+        for (int k = first; k < last; ++k) {
+            const float pos = fmaf((float)k + 0.5f, step, iv.lower);
+            const float p0 = fmaf(pos, d[0], o[0]);
+            const float p1 = fmaf(pos, d[1], o[1]);
+            const float p2 = fmaf(pos, d[2], o[2]);
+            if (!inside32(P, p0, p1, p2)) continue;
+            Cell32 c;
+            cell32_inside(P, p0, p1, p2, c);
+            float wt[8];
+            weights32(c, wt);
+            const int voxel = (c.lz * P.ny + c.ly) * P.nx + c.lx;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                const float inc = P.volume_f64 ? volume_factor * wt[kk] : scatter_increment(volume_factor, wt[kk]);
+                bad |= !isfinite(inc);
+                if (inc != 0.0f) unsafeAtomicAdd(&acc[voxel + corner_offset(P, kk)], (double)inc);
             }
         }
     }
@@ -2061,6 +2335,10 @@ void configure_tiles(Params &P, int vb) {
 // only 16-bit storage (blocked compensated sums, no FP32 counterpart)
 // segments the pose; FP32 storage runs it unsegmented in the pipelined loop.
 // DPT_AMD_VJP_SEGMENTS overrides the count (1..8; not the two-chunk floor).
+// A launch of at most 8192 waves (8 per SIMD): one whose sample loops expose
+// their memory latency.
+bool small_launch(const Params &P) { return (long long)P.total_tiles * (kBlock / 64) * 2 <= 16384; }
+
 void configure_segments(Params &P) {
     const long long waves = (long long)P.total_tiles * (kBlock / 64);
     int g = 1;
@@ -2139,6 +2417,20 @@ int launch_forward(dpt_plan *p, const Params &P, const void *field, void *out, b
     if (p->arithmetic == DPT_ARITH_FP32) {
         float *o = static_cast<float *>(out);
         constexpr bool HALF = !std::is_same<T, float>::value;
+        if constexpr (!HALF) {
+            if (small_launch(P)) {
+                if (cells)
+                    hipLaunchKernelGGL((forward32_kernel<T, false, kLayoutCells, true>), grid, block, 0, s,
+                                       cells_params(p, P), static_cast<const T *>(p->d_cells), o);
+                else if (P.nx > 1)
+                    hipLaunchKernelGGL((forward32_kernel<T, false, kLayoutPair, true>), grid, block, 0, s, P, f, o);
+                else
+                    hipLaunchKernelGGL((forward32_kernel<T, false, kLayoutScalar, true>), grid, block, 0, s, P, f,
+                                       o);
+                HIP_TRY(hipGetLastError());
+                return DPT_OK;
+            }
+        }
         if (cells)
             hipLaunchKernelGGL((forward32_kernel<T, HALF, kLayoutCells>), grid, block, 0, s, cells_params(p, P),
                                static_cast<const T *>(p->d_cells), o);
@@ -2182,9 +2474,13 @@ void launch_vjp32(dpt_plan *p, const Params &P, const T *f, const float *sd, boo
     else if (pose)
         hipLaunchKernelGGL((vjp32_kernel<T, false, true, C, HALF, LAYOUT>), grid, block, 0, s, P, f, sd, p->d_acc,
                            partials);
-    else if constexpr (LAYOUT != kLayoutCells)  // a volume-only VJP gathers nothing
-        hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd,
-                           p->d_acc, partials);
+    else if constexpr (LAYOUT != kLayoutCells) {  // a volume-only VJP gathers nothing
+        if (P.slab_chunks && (C == 6 || C == 12))
+            hipLaunchKernelGGL(vol32_slab_kernel, grid, block, lds, s, P, sd, p->d_acc);
+        else
+            hipLaunchKernelGGL((vjp32_kernel<T, true, false, C, HALF, LAYOUT>), grid, block, lds, s, P, f, sd,
+                               p->d_acc, partials);
+    }
 }
 
 template <typename T, int C>

@@ -268,6 +268,37 @@ def test_fp64_slab_chunks_match_oracle(cap, chunk, mode, monkeypatch):
         assert _rel(p_gpu, p_ref) <= 1e-10
 
 
+@pytest.mark.parametrize("segments", [None, "1", "4"])
+@pytest.mark.parametrize("cap,chunk", [(None, None), ("512", None), (None, "1"), ("512", "7"), (None, "64")])
+def test_fp32_slab_chunks_match_oracle(cap, chunk, segments, monkeypatch):
+    """The FP32 volume adjoint chunked by slabs of grid samples (forced on):
+    volume-only and, where the launch segments, beside the pipelined pose
+    adjoint; from the smallest brick (device-atomic fallback chunks) up,
+    chunk lengths 1 to 64, one to four sample segments."""
+    print(f"case fp32 slabs cap={cap} chunk={chunk} segments={segments}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", "1")
+    if cap is not None:
+        monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
+    if chunk is not None:
+        monkeypatch.setenv("DPT_AMD_CHUNK", chunk)
+    if segments is not None:
+        monkeypatch.setenv("DPT_AMD_VJP_SEGMENTS", segments)
+    grid, panels, values = _cone(n=24, views=5, rows=20, cols=28, seed=6)
+    pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    ref = O.Scene(grid, 512, panels=panels)
+    gpu = NativeScene(grid, 512, panels=panels)
+    assert gpu.plan.layout()["slab_chunks"]
+    image, _ = ref.forward(values, pose)
+    seeds = np.random.default_rng(12).uniform(-1.0, 1.0, image.size).astype(np.float32)
+    for pose_grad in (False, True):
+        g_ref, p_ref, _, _ = ref.vjp(values, seeds, pose, pose=pose_grad)
+        g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose, pose_grad=pose_grad)
+        assert status.tolist() == [0] * len(status)
+        assert _rel(g_gpu, g_ref) <= 1e-5, (pose_grad, _rel(g_gpu, g_ref))
+        if pose_grad:
+            assert _rel(p_gpu, p_ref) <= 1e-12
+
+
 def test_plans_choose_slab_chunks_for_a_silhouette_heavy_geometry():
     """A 512 x 512 x 139 chest grid at 0.76 x 0.76 x 2.5 mm under a 512^2
     panel at 2.67 mm (the reference's gather workload, benchmarks/gather.py)
