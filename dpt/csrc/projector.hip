@@ -2041,24 +2041,33 @@ __global__ __launch_bounds__(kBlock) void pose_reduce_kernel(const double *__res
                                                              int tiles_per_view, double *grad_pose,
                                                              int accumulate, int *status) {
     const int view = blockIdx.x;
-    __shared__ double s[kBlock];
+    // every component at once: per-thread strided sums, then one pairwise
+    // tree per component (the order of a component-by-component pass)
+    __shared__ double s[C][kBlock];
     const double *base = partials + (size_t)view * tiles_per_view * C;
-    for (int c = 0; c < C; ++c) {
-        double v = 0.0;
-        for (int t = threadIdx.x; t < tiles_per_view; t += kBlock) v += base[(size_t)t * C + c];
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
-            if (threadIdx.x < stride) s[threadIdx.x] += s[threadIdx.x + stride];
-            __syncthreads();
+    double v[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = 0.0;
+    for (int t = threadIdx.x; t < tiles_per_view; t += kBlock) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[c] += base[(size_t)t * C + c];
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) s[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int stride = kBlock / 2; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + stride];
         }
-        if (threadIdx.x == 0) {
-            double result = s[0];
-            if (accumulate) result += grad_pose[(size_t)view * C + c];
-            grad_pose[(size_t)view * C + c] = result;
-            if (!isfinite(result)) atomicOr(&status[view], DPT_STATUS_RANGE);
-        }
         __syncthreads();
+    }
+    if (threadIdx.x < C) {
+        const int c = threadIdx.x;
+        double result = s[c][0];
+        if (accumulate) result += grad_pose[(size_t)view * C + c];
+        grad_pose[(size_t)view * C + c] = result;
+        if (!isfinite(result)) atomicOr(&status[view], DPT_STATUS_RANGE);
     }
 }
 
