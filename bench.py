@@ -961,9 +961,9 @@ def roofline_report(args, fwd_ms: float, vjp_ms: float, effective: int, half: bo
 
     gather_b = 16 if half else GATHER_B
     if vjp_ms >= fwd_ms:
-        kernel, ms, per_sample = "vjp32_kernel<float, true, true, 12, false, 2>", vjp_ms, gather_b + SCATTER_B
+        kernel, ms, per_sample = "vjp32_kernel<float, true, true, 12, false, 2, false>", vjp_ms, gather_b + SCATTER_B
     else:
-        kernel, ms, per_sample = "forward32_kernel<float, false, 2>", fwd_ms, gather_b
+        kernel, ms, per_sample = "forward32_kernel<float, false, 2, false>", fwd_ms, gather_b
     digest = sources_digest()
     recorded, counters = None, None
     if COUNTERS.exists():
