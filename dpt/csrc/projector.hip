@@ -994,8 +994,11 @@ __global__ __launch_bounds__(kBlock) void vol32_slab_kernel(Params P, const floa
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
     const int mine = ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2);
+    int my_votes[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) my_votes[a] = __popcll(__builtin_amdgcn_ballot_w64(summed && mine == a));
     if (lane == 0)
-        for (int a = 0; a < 3; ++a) bound_red[wv][a] = __popcll(__builtin_amdgcn_ballot_w64(summed && mine == a));
+        for (int a = 0; a < 3; ++a) bound_red[wv][a] = my_votes[a];
     __syncthreads();
     int votes[3];
 #pragma unroll

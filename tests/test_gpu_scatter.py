@@ -55,12 +55,18 @@ def _volume(gpu: NativeScene, values, seeds, *, joint=False, fp64=False, initial
     return gv.cpu().numpy(), status
 
 
+# "1": the volume adjoint by slab chunks (vol32_slab_kernel), forced on
+SLABS = ["0", "1"]
+
+
+@pytest.mark.parametrize("slabs", SLABS)
 @pytest.mark.parametrize("joint", [False, True])
 @pytest.mark.parametrize("seed_scale", [1.0, 1000.0, 1e-33])
 @pytest.mark.parametrize("samples", [2, 7, 1024])
 @pytest.mark.parametrize("direction", scenes.SHARED_FACE_DIRECTIONS)
-def test_shared_face_walks_match_the_per_sample_scatter(direction, samples, seed_scale, joint):
-    print(f"case shared face {direction} S={samples} scale={seed_scale} joint={joint}", flush=True)
+def test_shared_face_walks_match_the_per_sample_scatter(direction, samples, seed_scale, joint, slabs, monkeypatch):
+    print(f"case shared face {direction} S={samples} scale={seed_scale} joint={joint} slabs={slabs}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", slabs)
     grid, sources, endpoints, values = scenes.shared_face_scene(direction)
     seeds = (seed_scale * np.asarray([0.7, -0.3, 0.0])).astype(np.float32)
     ref = O.Scene(grid, samples, rays=(sources, endpoints))
@@ -72,10 +78,12 @@ def test_shared_face_walks_match_the_per_sample_scatter(direction, samples, seed
     np.testing.assert_allclose(actual, expected, rtol=2e-5, atol=3e-7 * seed_scale)
 
 
+@pytest.mark.parametrize("slabs", SLABS)
 @pytest.mark.parametrize("joint", [False, True])
 @pytest.mark.parametrize("sign", [-1.0, 1.0])
-def test_subnormal_increments_give_exact_zeros(sign, joint):
-    print(f"case subnormal sign={sign} joint={joint}", flush=True)
+def test_subnormal_increments_give_exact_zeros(sign, joint, slabs, monkeypatch):
+    print(f"case subnormal sign={sign} joint={joint} slabs={slabs}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", slabs)
     grid, source, target, values = scenes.subnormal_scene()
     seeds = np.asarray([sign * 1e-35], dtype=np.float32)
     gpu = NativeScene(grid, 1024, rays=(source, target))
@@ -90,10 +98,12 @@ def test_subnormal_increments_give_exact_zeros(sign, joint):
     np.testing.assert_allclose(wide, unflushed, rtol=2e-5, atol=0)
 
 
+@pytest.mark.parametrize("slabs", SLABS)
 @pytest.mark.parametrize("reverse", [False, True])
 @pytest.mark.parametrize("sign", [-1.0, 1.0])
-def test_overflow_scale_seeds_accumulate_finitely_onto_an_opposing_destination(sign, reverse):
-    print(f"case overflow sign={sign} reverse={reverse}", flush=True)
+def test_overflow_scale_seeds_accumulate_finitely_onto_an_opposing_destination(sign, reverse, slabs, monkeypatch):
+    print(f"case overflow sign={sign} reverse={reverse} slabs={slabs}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", slabs)
     grid, source, target, values = scenes.overflow_scene(reverse)
     seeds = np.asarray([sign * 2e38], dtype=np.float32)
     expected, _, _, _ = O.Scene(grid, 1024, rays=(source, target)).vjp(values, seeds, pose=False)
@@ -107,9 +117,11 @@ def test_overflow_scale_seeds_accumulate_finitely_onto_an_opposing_destination(s
 
 
 @pytest.mark.parametrize("direction", [(1, 0, 0), (0, -1, 0), (-1, -1, -1)])
-def test_fp32_sampling_into_an_fp64_destination(direction):
+@pytest.mark.parametrize("slabs", SLABS)
+def test_fp32_sampling_into_an_fp64_destination(direction, slabs, monkeypatch):
     """Unflushed RN32 increments summed in FP64 (the FP64-destination scatter)."""
-    print(f"case fp64 destination {direction}", flush=True)
+    print(f"case fp64 destination {direction} slabs={slabs}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_CHUNKS", slabs)
     grid, sources, endpoints, values = scenes.shared_face_scene(direction)
     seeds = np.asarray([0.7, -0.3, 0.0], dtype=np.float32)
     expected = O.explicit_fp32_scatter(grid, list(zip(sources, endpoints)), scenes.IDENTITY12, 97, seeds, flush=False)
