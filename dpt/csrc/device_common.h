@@ -44,6 +44,7 @@ struct Params {
     int segments;            // FP32 VJP: workgroups per tile, each over its own range of samples
     int base_tiles;          // tiles of one segment (total_tiles / segments)
     int segment_samples;     // samples per segment (a multiple of chunk_samples)
+    int slab_fallback;       // FP32 VJP: a chunk that overflows at the shortest length switches the workgroup to slab chunks
     int wt_x, wt_y, wt_per_view;  // wave tiles (8x8 pixels, or 64 rays) per view
     const double *panels;    // views x 12
     const double *ray_src;   // views x pixels x 3

@@ -739,14 +739,99 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 if (k0 + j < last) consume(j, k0 + j);  // uniform
         }
     } else {
+    // One sample of the walk: the cell, its corner values on a cell change
+    // (with the previous cell's flush), the volume cell sums and the pose
+    // sums.  SLAB: per-lane sample indices (table reads by vector loads) and
+    // support tests on every sample.
+    auto take = [&](auto slab, int k, bool checked, const Brick &B, bool use_lds, int &cur, float *v) {
+        constexpr bool SLAB = decltype(slab)::value;
+        const float pos = fmaf(SLAB ? (float)k + 0.5f : sample_centre(P, k), step, iv.lower);
+        const float p0 = fmaf(pos, d[0], o[0]);
+        const float p1 = fmaf(pos, d[1], o[1]);
+        const float p2 = fmaf(pos, d[2], o[2]);
+        Cell32 c;
+        cell32_inside(P, p0, p1, p2, c);
+        float factor = loop_factor, live = 1.0f;
+        if (SLAB || checked) {  // outside samples add nothing
+            const bool inside = inside32(P, p0, p1, p2);
+            c.fx = inside ? c.fx : 0.0f;
+            c.fy = inside ? c.fy : 0.0f;
+            c.fz = inside ? c.fz : 0.0f;
+            c.slx = inside ? c.slx : 0.0f;
+            c.sly = inside ? c.sly : 0.0f;
+            c.slz = inside ? c.slz : 0.0f;
+            factor = inside ? factor : 0.0f;
+            live = inside ? 1.0f : 0.0f;
+        }
+        if (c.base != cur) {
+            if constexpr (POSE) load8<T, LAYOUT>(P, field, c.base, v);
+            if constexpr (VOL) {
+                if (cs.x >= 0) {
+                    if (use_lds)
+                        flush_lds(B, cs, brick);
+                    else
+                        flush_device(P, cs, acc);
+                }
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) cs.s[kk] = 0.0f;
+                // lanes that add elsewhere keep an empty cell
+                cs.x = summed ? c.lx : -1;
+                cs.y = c.ly;
+                cs.z = c.lz;
+            }
+            cur = c.base;
+        }
+        float wt[8];
+        weights32(c, wt);
+        if constexpr (VOL) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) cs.s[kk] = fmaf(wt[kk], factor, cs.s[kk]);
+        }
+        if constexpr (POSE) {
+            float g[3];
+            // an outside sample's slopes are zero (above); its value
+            // is zeroed here
+            const float value = interpolate32(wt, v) * live;
+            slopes32(c, v, g);
+            mean.add(value);
+            dor0.add(g[0]);
+            dor1.add(g[1]);
+            dor2.add(g[2]);
+            // contracted by the reference's code generator (fuse_fp)
+            dd0.fma_add(pos, g[0]);
+            dd1.fma_add(pos, g[1]);
+            dd2.fma_add(pos, g[2]);
+            const float alpha = SLAB ? P.alpha[k] : sample_alpha(P, k);
+            const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
+            shl.fma_add(1.0f - alpha, along);
+            shu.fma_add(alpha, along);
+        }
+    };
+    auto fold_all = [&]() {
+        if constexpr (POSE && COMPENSATED) {
+            mean.fold();
+            dor0.fold();
+            dor1.fold();
+            dor2.fold();
+            dd0.fold();
+            dd1.fold();
+            dd2.fold();
+            shl.fold();
+            shu.fold();
+        }
+    };
     // Chunks of P.chunk_samples samples; a chunk whose brick overflows LDS is
-    // halved (down to kMinChunk samples) until it fits, and only then flushes
-    // to device atomics.  Chunk bounds are uniform over the workgroup.
+    // halved (down to kMinChunk samples) until it fits; if it still overflows
+    // the workgroup walks its remaining samples by slab chunks (below), or,
+    // without the fallback, flushes that chunk to device atomics.  Chunk
+    // bounds are uniform over the workgroup.
+    int switch_at = -1;
     for (int k0 = first, k1 = first; k0 < last; k0 = k1) {
         Brick B{};
         bool use_lds = false;
         k1 = min(k0 + P.chunk_samples, last);
         if constexpr (VOL) {
+            bool overflow = false;
             for (int len = P.chunk_samples;; len >>= 1) {
                 k1 = min(k0 + len, last);
                 // ---- bound the cells of [k0, k1) over the workgroup
@@ -766,10 +851,13 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 // x-pair and cell layouts run only on grids with nx > 1: a
                 // compile-time unit x stride lets the LDS atomics of
                 // x-adjacent corners use the instruction's offset field
-                bool overflow = false;
                 use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B, &overflow, true);
                 if (!overflow || len <= kMinChunk) break;
                 __syncthreads();  // the next bound rewrites bound_red
+            }
+            if (overflow && P.slab_fallback) {  // uniform
+                switch_at = k0;
+                break;
             }
         }
         if (active) {
@@ -778,81 +866,9 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
             // per-sample support tests only for waves with a ray whose chunk
             // ends leave the support (uniform over the wave)
             const bool checked = __builtin_amdgcn_ballot_w64(!span_inside(P, o, d, iv.lower, step, k0, k1)) != 0;
-            for (int k = k0; k < k1; ++k) {
-                const float pos = fmaf(sample_centre(P, k), step, iv.lower);
-                const float p0 = fmaf(pos, d[0], o[0]);
-                const float p1 = fmaf(pos, d[1], o[1]);
-                const float p2 = fmaf(pos, d[2], o[2]);
-                Cell32 c;
-                cell32_inside(P, p0, p1, p2, c);
-                float factor = loop_factor, live = 1.0f;
-                if (checked) {  // uniform: outside samples add nothing
-                    const bool inside = inside32(P, p0, p1, p2);
-                    c.fx = inside ? c.fx : 0.0f;
-                    c.fy = inside ? c.fy : 0.0f;
-                    c.fz = inside ? c.fz : 0.0f;
-                    c.slx = inside ? c.slx : 0.0f;
-                    c.sly = inside ? c.sly : 0.0f;
-                    c.slz = inside ? c.slz : 0.0f;
-                    factor = inside ? factor : 0.0f;
-                    live = inside ? 1.0f : 0.0f;
-                }
-                if (c.base != cur) {
-                    if constexpr (POSE) load8<T, LAYOUT>(P, field, c.base, v);
-                    if constexpr (VOL) {
-                        if (cs.x >= 0) {
-                            if (use_lds)
-                                flush_lds(B, cs, brick);
-                            else
-                                flush_device(P, cs, acc);
-                        }
-#pragma unroll
-                        for (int kk = 0; kk < 8; ++kk) cs.s[kk] = 0.0f;
-                        // lanes that add elsewhere keep an empty cell
-                        cs.x = summed ? c.lx : -1;
-                        cs.y = c.ly;
-                        cs.z = c.lz;
-                    }
-                    cur = c.base;
-                }
-                float wt[8];
-                weights32(c, wt);
-                if constexpr (VOL) {
-#pragma unroll
-                    for (int kk = 0; kk < 8; ++kk) cs.s[kk] = fmaf(wt[kk], factor, cs.s[kk]);
-                }
-                if constexpr (POSE) {
-                    float g[3];
-                    // an outside sample's slopes are zero (above); its value
-                    // is zeroed here
-                    const float value = interpolate32(wt, v) * live;
-                    slopes32(c, v, g);
-                    mean.add(value);
-                    dor0.add(g[0]);
-                    dor1.add(g[1]);
-                    dor2.add(g[2]);
-                    // contracted by the reference's code generator (fuse_fp)
-                    dd0.fma_add(pos, g[0]);
-                    dd1.fma_add(pos, g[1]);
-                    dd2.fma_add(pos, g[2]);
-                    const float alpha = sample_alpha(P, k);
-                    const float along = fmaf(g[2], d[2], fmaf(g[1], d[1], g[0] * d[0]));
-                    shl.fma_add(1.0f - alpha, along);
-                    shu.fma_add(alpha, along);
-                }
-            }
+            for (int k = k0; k < k1; ++k) take(std::false_type{}, k, checked, B, use_lds, cur, v);
         }
-        if constexpr (POSE && COMPENSATED) {
-            mean.fold();
-            dor0.fold();
-            dor1.fold();
-            dor2.fold();
-            dd0.fold();
-            dd1.fold();
-            dd2.fold();
-            shl.fold();
-            shu.fold();
-        }
+        fold_all();
         if constexpr (VOL) {
             if (cs.x >= 0) {
                 if (use_lds)
@@ -866,6 +882,120 @@ __global__ __launch_bounds__(kBlock) void vjp32_kernel(Params P, const T *__rest
                 drain_brick(P, B, brick, acc);
             }
             __syncthreads();
+        }
+    }
+    if constexpr (VOL) {
+        if (switch_at >= 0) {  // uniform
+            // ---- slab chunks over [switch_at, last): space cut into slabs of
+            // `width` grid samples along the workgroup's dominant axis; each
+            // lane walks, per slab, its samples whose axis coordinate lies in
+            // the slab.  Boundaries are integers and both sides of one use
+            // the same expression, so every sample falls in exactly one slab,
+            // and a lane's slabs come in its sample order (its pose sums keep
+            // their order).  The brick bound comes from each slab's own end
+            // samples.
+            __syncthreads();  // the last chunk bound is read
+            const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const float ad[3] = {fabsf(d[0]), fabsf(d[1]), fabsf(d[2])};
+            const int mine = ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2);
+            int my_votes[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) my_votes[a] = __popcll(__builtin_amdgcn_ballot_w64(active && mine == a));
+            if (lane == 0)
+                for (int a = 0; a < 3; ++a) bound_red[wv][a] = my_votes[a];
+            __syncthreads();
+            int votes[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                votes[a] = bound_red[0][a] + bound_red[1][a] + bound_red[2][a] + bound_red[3][a];
+            const int axis = votes[0] >= votes[1] && votes[0] >= votes[2] ? 0 : (votes[1] >= votes[2] ? 1 : 2);
+            const int n_axis = axis == 0 ? P.nx : (axis == 1 ? P.ny : P.nz);
+            const float oa = axis == 0 ? o[0] : (axis == 1 ? o[1] : o[2]);
+            const float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+            int slo = INT_MAX, snhi = INT_MAX, nadv = INT_MAX;
+            if (active) {
+                const float q0 = fmaf(fmaf((float)switch_at + 0.5f, step, iv.lower), da, oa);
+                const float q1 = fmaf(fmaf((float)(last - 1) + 0.5f, step, iv.lower), da, oa);
+                slo = (int)floorf(fminf(fmaxf(fminf(q0, q1), -2.0f), (float)n_axis + 1.0f));
+                snhi = -(int)ceilf(fminf(fmaxf(fmaxf(q0, q1), -2.0f), (float)n_axis + 1.0f));
+                nadv = -(int)fminf(fmaxf(ceilf(step * fabsf(da) * 64.0f), 1.0f), 1048576.0f);
+            }
+            slo = wave_min(slo);
+            snhi = wave_min(snhi);
+            nadv = wave_min(nadv);
+            __syncthreads();  // every wave has read the votes
+            if (lane == 0) {
+                bound_red[wv][3] = slo;
+                bound_red[wv][4] = snhi;
+                bound_red[wv][5] = nadv;
+            }
+            __syncthreads();
+            int slab_lo = min(min(bound_red[0][3], bound_red[1][3]), min(bound_red[2][3], bound_red[3][3]));
+            const int slab_hi = -min(min(bound_red[0][4], bound_red[1][4]), min(bound_red[2][4], bound_red[3][4]));
+            const int adv = -min(min(bound_red[0][5], bound_red[1][5]), min(bound_red[2][5], bound_red[3][5]));
+            __syncthreads();  // bound_red is reused by the slab bounds
+            int width = 1, count = 1;
+            if (slab_lo == INT_MAX) {
+                slab_lo = 0;
+            } else {
+                width = max(1, (P.chunk_samples * adv + 63) / 64);
+                count = max(1, (slab_hi - slab_lo + width - 1) / width);
+            }
+            for (int ch = 0; ch < count; ++ch) {
+                int k0 = switch_at, k1 = switch_at;
+                if (active) {
+                    if (da == 0.0f) {
+                        const int own = min(max((int)floorf((oa - (float)slab_lo) / (float)width), 0), count - 1);
+                        k1 = ch == own ? last : switch_at;
+                    } else {
+                        // t(b): the sample index where the coordinate crosses b
+                        const int js = da > 0.0f ? ch : count - 1 - ch;
+                        const float b0 = (float)(slab_lo + js * width), b1 = (float)(slab_lo + (js + 1) * width);
+                        const float t0 = ((b0 - oa) / da - iv.lower) / step - 0.5f;
+                        const float t1 = ((b1 - oa) / da - iv.lower) / step - 0.5f;
+                        const float lo_k = (float)switch_at, hi_k = (float)last;
+                        if (da > 0.0f) {
+                            k0 = ch == 0 ? switch_at : (int)fminf(fmaxf(ceilf(t0), lo_k), hi_k);
+                            k1 = ch == count - 1 ? last : (int)fminf(fmaxf(ceilf(t1), lo_k), hi_k);
+                        } else {
+                            k0 = ch == 0 ? switch_at : (int)fminf(fmaxf(floorf(t1) + 1.0f, lo_k), hi_k);
+                            k1 = ch == count - 1 ? last : (int)fminf(fmaxf(floorf(t0) + 1.0f, lo_k), hi_k);
+                        }
+                        k1 = max(k1, k0);
+                    }
+                }
+                int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, nhi[3] = {INT_MAX, INT_MAX, INT_MAX};
+                if (summed && k1 > k0) {
+                    const float pa = fmaf((float)k0 + 0.5f, step, iv.lower);
+                    const float pb = fmaf((float)(k1 - 1) + 0.5f, step, iv.lower);
+                    int la[3], lb[3];
+                    lower_cell32(P, fmaf(pa, d[0], o[0]), fmaf(pa, d[1], o[1]), fmaf(pa, d[2], o[2]), la);
+                    lower_cell32(P, fmaf(pb, d[0], o[0]), fmaf(pb, d[1], o[1]), fmaf(pb, d[2], o[2]), lb);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = min(la[a], lb[a]);
+                        nhi[a] = -max(la[a], lb[a]);
+                    }
+                }
+                Brick B{};
+                const bool use_lds = chunk_brick(P, bound_red, lo, nhi, LAYOUT != kLayoutScalar, B, nullptr, true);
+                if (active) {
+                    float v[8];
+                    int cur = -1;
+                    for (int k = k0; k < k1; ++k) take(std::true_type{}, k, true, B, use_lds, cur, v);
+                }
+                fold_all();
+                if (cs.x >= 0) {
+                    if (use_lds)
+                        flush_lds(B, cs, brick);
+                    else
+                        flush_device(P, cs, acc);
+                }
+                cs.reset();
+                __syncthreads();
+                if (use_lds) drain_brick(P, B, brick, acc);
+                __syncthreads();
+            }
         }
     }
     }
@@ -2334,6 +2464,8 @@ void configure_tiles(Params &P, int vb) {
     P.segments = 1;
     P.base_tiles = P.total_tiles;
     P.segment_samples = P.samples;
+    P.slab_fallback = 1;
+    if (const char *fb = std::getenv("DPT_AMD_SLAB_FALLBACK")) P.slab_fallback = std::atoi(fb) != 0;
 }
 
 // Sample-range segments of the FP32 VJP: a launch of few tiles (one view of

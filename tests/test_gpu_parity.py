@@ -213,13 +213,17 @@ def test_half_storage_pose_segments_stay_near_the_fp64_vjp(monkeypatch):
     assert max(errors) <= 1.5 * _rel(p32, p64) + 1e-7, (errors, _rel(p32, p64))
 
 
+@pytest.mark.parametrize("fallback", ["1", "0"])
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 @pytest.mark.parametrize("cap,chunk", [("512", None), ("512", "7"), (None, "1"), (None, "64"), ("8192", "16")])
-def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch):
+def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, fallback, monkeypatch):
     """LDS bricks of every size down to the smallest capacity (512 entries),
-    where most chunks overflow and take the device-atomic path; chunk
-    lengths from 1 to 64 samples."""
-    print(f"case bricks cap={cap} chunk={chunk} {precision}", flush=True)
+    where most chunks overflow: the FP32 kernel then walks the rest of the
+    workgroup's samples by slab chunks (fallback "1", the default) or flushes
+    the chunk to device atomics ("0"); chunk lengths from 1 to 64 samples.
+    The pose sums keep their order either way."""
+    print(f"case bricks cap={cap} chunk={chunk} {precision} fallback={fallback}", flush=True)
+    monkeypatch.setenv("DPT_AMD_SLAB_FALLBACK", fallback)
     if cap is not None:
         monkeypatch.setenv("DPT_AMD_BRICK_CAP", cap)
     if chunk is not None:
@@ -237,7 +241,12 @@ def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, monkeypatch)
     g_gpu, p_gpu, status = gpu.vjp(values, seeds, pose)
     assert status.tolist() == [0] * len(status)
     assert _rel(g_gpu, g_ref) <= (1e-5 if precision == "float32" else 1e-7)
-    assert _rel(p_gpu, p_ref) <= (1e-5 if precision == "float32" else 1e-10)
+    # FP32: each ray's pose sums are the oracle's sequential FP32 sums
+    assert _rel(p_gpu, p_ref) <= (1e-12 if precision == "float32" else 1e-10)
+    if precision == "float32":
+        g_vol, _, status = gpu.vjp(values, seeds, pose, pose_grad=False)
+        assert status.tolist() == [0] * len(status)
+        assert _rel(g_vol, g_ref) <= 1e-5
 
 
 @pytest.mark.parametrize("mode", ["both", "volume"])
