@@ -249,6 +249,30 @@ def test_volume_vjp_brick_paths_match_oracle(cap, chunk, precision, fallback, mo
         assert _rel(g_vol, g_ref) <= 1e-5
 
 
+@pytest.mark.parametrize("storage", ["bfloat16", "float16"])
+def test_half_storage_slab_fallback_matches_index_chunks(storage, monkeypatch):
+    """16-bit storage (blocked compensated pose sums) through the slab
+    fallback: with the smallest brick most workgroups switch to slab chunks,
+    whose compensation blocks differ from the index chunks' (device-atomic
+    flushes); volume and pose gradients agree to FP32 rounding."""
+    print(f"case half slab fallback {storage}", flush=True)
+    monkeypatch.setenv("DPT_AMD_BRICK_CAP", "512")
+    grid, panels, values = _cone(n=24, views=5, rows=20, cols=28, seed=5)
+    pose = scenes.compose(scenes.IDENTITY12, (0.5, -0.7, 0.3, 0.02, -0.015, 0.03))
+    ref = O.Scene(grid, 97, panels=panels)
+    image, _ = ref.forward(values, pose)
+    seeds = np.random.default_rng(9).uniform(-1.0, 1.0, image.size).astype(np.float32)
+    out = {}
+    for fallback in ("1", "0"):
+        monkeypatch.setenv("DPT_AMD_SLAB_FALLBACK", fallback)
+        gpu = NativeScene(grid, 97, panels=panels, storage=storage)
+        g, p_, status = gpu.vjp(values, seeds, pose)
+        assert status.tolist() == [0] * len(status)
+        out[fallback] = (g, p_)
+    assert _rel(out["1"][0], out["0"][0]) <= 1e-6
+    assert _rel(out["1"][1], out["0"][1]) <= 1e-6
+
+
 @pytest.mark.parametrize("mode", ["both", "volume"])
 @pytest.mark.parametrize("cap,chunk", [(None, None), ("512", None), (None, "1"), ("512", "7"), (None, "64")])
 def test_fp64_slab_chunks_match_oracle(cap, chunk, mode, monkeypatch):
