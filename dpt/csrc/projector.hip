@@ -1422,7 +1422,7 @@ struct Slabs {
 __device__ __forceinline__ double pick3(const double *v, int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
 __device__ __forceinline__ Slabs slab_setup(const Params &P, bool active, const Ray &r, const Interval64 &iv,
-                                            double step, int (*red)[6]) {
+                                            double step, int (*red)[6], int from = 0) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double ad[3] = {fabs(r.d[0]), fabs(r.d[1]), fabs(r.d[2])};
     const int mine = ad[0] >= ad[1] && ad[0] >= ad[2] ? 0 : (ad[1] >= ad[2] ? 1 : 2);
@@ -1440,7 +1440,7 @@ __device__ __forceinline__ Slabs slab_setup(const Params &P, bool active, const 
     int lo = INT_MAX, nhi = INT_MAX, nadv = INT_MAX;
     if (active) {
         const double oa = pick3(r.o, axis), da = pick3(r.d, axis);
-        const double q0 = oa + (iv.lower + 0.5 * step) * da;
+        const double q0 = oa + (iv.lower + ((double)from + 0.5) * step) * da;
         const double q1 = oa + (iv.lower + ((double)P.samples - 0.5) * step) * da;
         lo = (int)floor(fmin(fmax(fmin(q0, q1), -2.0), (double)n + 1.0));
         nhi = -(int)ceil(fmin(fmax(fmax(q0, q1), -2.0), (double)n + 1.0));
@@ -1474,16 +1474,16 @@ __device__ __forceinline__ Slabs slab_setup(const Params &P, bool active, const 
     return out;
 }
 
-// Sample range [k0, k1) of this lane in chunk j.
+// Sample range [k0, k1) of this lane in chunk j, over samples [from, S).
 __device__ __forceinline__ void slab_range(const Params &P, bool active, const Ray &r, const Interval64 &iv,
-                                           double step, const Slabs &sl, int j, int &k0, int &k1) {
-    k0 = k1 = 0;
+                                           double step, const Slabs &sl, int j, int &k0, int &k1, int from = 0) {
+    k0 = k1 = from;
     if (!active) return;
     const double oa = pick3(r.o, sl.axis), da = pick3(r.d, sl.axis);
-    const double S = (double)P.samples;
+    const double S = (double)P.samples, F = (double)from;
     if (da == 0.0) {
         const int own = min(max((int)floor((oa - (double)sl.lo) / (double)sl.width), 0), sl.count - 1);
-        k1 = j == own ? P.samples : 0;
+        k1 = j == own ? P.samples : from;
         return;
     }
     // t(b) = ((b - o) / d - lower) / step - 0.5, monotone in b: the sample
@@ -1494,18 +1494,21 @@ __device__ __forceinline__ void slab_range(const Params &P, bool active, const R
     const double t1 = ((b1 - oa) / da - iv.lower) / step - 0.5;
     if (da > 0.0) {
         // coordinate(k) >= b  <=>  k >= t(b)
-        k0 = j == 0 ? 0 : (int)fmin(fmax(ceil(t0), 0.0), S);
-        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(ceil(t1), 0.0), S);
+        k0 = j == 0 ? from : (int)fmin(fmax(ceil(t0), F), S);
+        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(ceil(t1), F), S);
     } else {
         // coordinate(k) < b  <=>  k > t(b)
-        k0 = j == 0 ? 0 : (int)fmin(fmax(floor(t1) + 1.0, 0.0), S);
-        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(floor(t0) + 1.0, 0.0), S);
+        k0 = j == 0 ? from : (int)fmin(fmax(floor(t1) + 1.0, F), S);
+        k1 = j == sl.count - 1 ? P.samples : (int)fmin(fmax(floor(t0) + 1.0, F), S);
     }
     k1 = max(k1, k0);
 }
 
+// The joint form is held to three waves per SIMD (168 VGPRs, about 100 bytes
+// of spills, mostly the slab fallback's state): left free it takes 189 and
+// two waves, and ran the full scan's FP64 joint VJP in 736 against 590 ms.
 template <typename T, bool VOL, bool POSE, int C, bool SLABS = false>
-__global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__restrict__ field,
+__global__ __launch_bounds__(kBlock, (VOL && POSE) ? 3 : 1) void vjp64_kernel(Params P, const T *__restrict__ field,
                                                        const double *__restrict__ seeds,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ partials) {
@@ -1561,14 +1564,19 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
     }
     double v[8];
     int cur = -1;  // corners cached while consecutive samples share a cell
-    // SLABS (volume adjoint only, chosen per plan): chunks by slabs of space
+    // SLABS (volume adjoint only, chosen per plan): chunks by slabs of space.
+    // Index chunks switch to slabs for the rest of the workgroup's samples
+    // when one overflows the brick (P.slab_fallback); each lane's slabs come
+    // in its sample order, so its sums keep their order.
     Slabs slabs{0, 0, 1, 1};
+    bool slab_mode = VOL && SLABS;
+    int from = 0;
     if constexpr (VOL && SLABS) slabs = slab_setup(P, active, r, iv, step, bound_red);
-    const int chunks = (VOL && SLABS) ? slabs.count : (P.samples + P.chunk_samples - 1) / P.chunk_samples;
+    int chunks = slab_mode ? slabs.count : (P.samples + P.chunk_samples - 1) / P.chunk_samples;
     for (int ch = 0; ch < chunks; ++ch) {
         int k0, k1;
-        if constexpr (VOL && SLABS) {
-            slab_range(P, active, r, iv, step, slabs, ch, k0, k1);
+        if (slab_mode) {
+            slab_range(P, active, r, iv, step, slabs, ch, k0, k1, from);
         } else {
             k0 = ch * P.chunk_samples;
             k1 = min(k0 + P.chunk_samples, P.samples);
@@ -1592,7 +1600,17 @@ __global__ __launch_bounds__(kBlock) void vjp64_kernel(Params P, const T *__rest
                     nhi[a] = -max(la[a], lb[a]);
                 }
             }
-            use_lds = chunk_brick(P, bound_red, lo, nhi, false, B);
+            bool overflow = false;
+            use_lds = chunk_brick(P, bound_red, lo, nhi, false, B, &overflow);
+            if (overflow && !slab_mode && P.slab_fallback) {  // uniform
+                __syncthreads();  // the chunk bound is read
+                from = k0;
+                slab_mode = true;
+                slabs = slab_setup(P, active, r, iv, step, bound_red, from);
+                chunks = slabs.count;
+                ch = -1;
+                continue;
+            }
         }
         if (active) {
             for (int k = k0; k < k1; ++k) {
